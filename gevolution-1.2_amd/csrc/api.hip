@@ -1,0 +1,694 @@
+// api.hip -- context lifecycle, data movement, the C ABI of include/gev_pm.h,
+// the host-side cycle driver (main.cpp:372-879) and the -DBENCHMARK-style
+// kernel timers.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "gev_internal.h"
+
+namespace gev {
+bool fft_size_supported(int N);
+
+int fail(gev_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int check_hip(gev_ctx *ctx, hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    return fail(ctx, GEV_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static hipEvent_t take_event(gev_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void timer_begin(gev_ctx *ctx, const char *name) {
+    if (!ctx->timing) return;
+    TimerRec r;
+    r.name = name;
+    r.beg = take_event(ctx);
+    r.end = take_event(ctx);
+    if (!r.beg || !r.end) return;
+    hipEventRecord(r.beg, ctx->stream);
+    ctx->pending.push_back(r);
+}
+
+void timer_end(gev_ctx *ctx) {
+    if (!ctx->timing || ctx->pending.empty()) return;
+    hipEventRecord(ctx->pending.back().end, ctx->stream);
+}
+
+static void timers_collect(gev_ctx *ctx) {
+    if (ctx->pending.empty()) return;
+    hipStreamSynchronize(ctx->stream);
+    for (auto &r : ctx->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.beg, r.end);
+        TimerAcc *a = nullptr;
+        for (auto &x : ctx->acc)
+            if (x.name == r.name) a = &x;
+        if (!a) {
+            ctx->acc.push_back(TimerAcc{r.name, 0, 0.0});
+            a = &ctx->acc.back();
+        }
+        a->calls++;
+        a->ms += ms;
+        ctx->event_pool.push_back(r.beg);
+        ctx->event_pool.push_back(r.end);
+    }
+    ctx->pending.clear();
+}
+
+// gridk2 / kshift tables exactly as gevolution.hpp:222-227 computes them
+static void host_ktables(int N, std::vector<double> &g, std::vector<double2> &ks) {
+    g.resize(N);
+    ks.resize(N);
+    for (int i = 0; i < N; i++) {
+        g[i] = 2. * (double)N * sin(M_PI * (double)i / (double)N);
+        double c = cos(M_PI * (double)i / (double)N), s = -sin(M_PI * (double)i / (double)N);
+        ks[i] = make_double2(g[i] * c, g[i] * s);
+        g[i] *= g[i];
+    }
+}
+
+static void host_twiddles(int N, std::vector<double2> &tw) {
+    tw.resize(N);
+    for (int t = 0; t < N; t++) {
+        long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)N;
+        tw[t] = make_double2((double)cosl(ang), -(double)sinl(ang));
+    }
+}
+
+struct Sizes {
+    int64_t n3, nk, cap, nblk_scan, nred;
+};
+
+static Sizes sizes_of(int N, int64_t cap) {
+    Sizes s;
+    s.n3 = (int64_t)N * N * N;
+    s.nk = (int64_t)kpitch(N) * N * N;
+    s.cap = cap < 1 ? 1 : cap;
+    s.nblk_scan = (s.n3 + 4095) / 4096;
+    s.nred = 8 + (s.n3 + 4095) / 4096 + 64;
+    return s;
+}
+
+static int64_t footprint(int N, int64_t cap) {
+    Sizes s = sizes_of(N, cap);
+    int64_t b = 0;
+    b += 12 * s.n3 * 8;             // phi, chi, source, Sij(6), Bi(3)
+    b += 11 * s.nk * 16;            // scalarFT, SijFT(6), BiFT(3), fft scratch
+    b += s.cap * (6 * 8 + 8 + 3 * 4);  // pos, q, id, cell, slot, perm
+    b += s.n3 * 4 + (s.n3 + 1) * 4 + s.nblk_scan * 4;
+    b += s.nred * 8 + 8;
+    b += N * (16 + 8 + 16);
+    return b;
+}
+
+static void free_all(gev_ctx *c) {
+    void *ptrs[] = {c->phi, c->chi, c->source, c->Sij, c->Bi, c->scalarFT, c->SijFT, c->BiFT,
+                    c->fft_tmp, c->twiddle, c->gridk2, c->kshift, c->pos, c->q, c->id, c->cell,
+                    c->slot, c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (c->host_pinned) hipHostFree(c->host_pinned);
+}
+
+// ---------------------------------------------------------------- cycle
+
+static double hconf(double a, double fourpiG, const gev_cosmology *c) {
+    // background.hpp:137-140 with bg_ncdm = 0
+    return sqrt((2. * fourpiG / 3.) *
+                (((c->Omega_cdm + c->Omega_b + 0.0) / a) + (c->Omega_Lambda * a * a) +
+                 (c->Omega_rad / a / a) +
+                 (c->Omega_fld * exp(3. * c->wa_fld * (a - 1.)) /
+                  pow(a, 1. + 3. * (c->w0_fld + c->wa_fld)))));
+}
+
+static void rk4bg(double &a, double fourpiG, const gev_cosmology *c, double dtau) {
+    // background.hpp:167-177
+    double k1a, k2a, k3a, k4a;
+    k1a = a * hconf(a, fourpiG, c);
+    k2a = (a + k1a * dtau / 2.) * hconf(a + k1a * dtau / 2., fourpiG, c);
+    k3a = (a + k2a * dtau / 2.) * hconf(a + k2a * dtau / 2., fourpiG, c);
+    k4a = (a + k3a * dtau) * hconf(a + k3a * dtau, fourpiG, c);
+    a += dtau * (k1a + 2. * k2a + 2. * k3a + k4a) / 6.;
+}
+
+}  // namespace gev
+
+using namespace gev;
+
+#define GEV_EXPORT extern "C" __attribute__((visibility("default")))
+
+GEV_EXPORT int64_t gev_ctx_footprint_bytes(const gev_box_desc *d) {
+    if (!d || d->ngrid < 2 || d->npart < 0) return -1;
+    return footprint(d->ngrid, d->npart);
+}
+
+GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
+    if (!out) return GEV_EINVAL;
+    *out = nullptr;
+    if (!d || d->npart < 0 || d->npart > 0xFFFFFFFFll) return GEV_EINVAL;
+    if (!fft_size_supported(d->ngrid)) return GEV_EINVAL;
+    gev_ctx *c = new (std::nothrow) gev_ctx();
+    if (!c) return GEV_ENOMEM;
+    c->N = d->ngrid;
+    c->P = kpitch(c->N);
+    c->cap = d->npart;
+    hipGetDevice(&c->device);
+    if (d->stream) {
+        c->stream = (hipStream_t)d->stream;
+    } else {
+        if (hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            return GEV_EHIP;
+        }
+        c->own_stream = true;
+    }
+    Sizes s = sizes_of(c->N, c->cap);
+    bool ok = true;
+    auto A = [&](void **p, size_t bytes) {
+        if (!ok) return;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            ok = false;
+        }
+    };
+    A((void **)&c->phi, s.n3 * 8);
+    A((void **)&c->chi, s.n3 * 8);
+    A((void **)&c->source, s.n3 * 8);
+    A((void **)&c->Sij, 6 * s.n3 * 8);
+    A((void **)&c->Bi, 3 * s.n3 * 8);
+    A((void **)&c->scalarFT, s.nk * 16);
+    A((void **)&c->SijFT, 6 * s.nk * 16);
+    A((void **)&c->BiFT, 3 * s.nk * 16);
+    A((void **)&c->fft_tmp, s.nk * 16);
+    A((void **)&c->twiddle, c->N * 16);
+    A((void **)&c->gridk2, c->N * 8);
+    A((void **)&c->kshift, c->N * 16);
+    A((void **)&c->pos, 3 * s.cap * 8);
+    A((void **)&c->q, 3 * s.cap * 8);
+    A((void **)&c->id, s.cap * 8);
+    A((void **)&c->cell, s.cap * 4);
+    A((void **)&c->slot, s.cap * 4);
+    A((void **)&c->perm, s.cap * 4);
+    A((void **)&c->count, s.n3 * 4);
+    A((void **)&c->start, (s.n3 + 1) * 4);
+    A((void **)&c->scan_part, s.nblk_scan * 4);
+    A((void **)&c->red_d, s.nred * 8);
+    A((void **)&c->red_u, 64);
+    if (ok && hipHostMalloc((void **)&c->host_pinned, 64) != hipSuccess) ok = false;
+    if (!ok) {
+        free_all(c);
+        if (c->own_stream) hipStreamDestroy(c->stream);
+        delete c;
+        return GEV_ENOMEM;
+    }
+    c->scan_nblk = s.nblk_scan;
+    std::vector<double> g;
+    std::vector<double2> ks, tw;
+    host_ktables(c->N, g, ks);
+    host_twiddles(c->N, tw);
+    int rc = 0;
+    rc = rc ? rc : check_hip(c, hipMemcpy(c->gridk2, g.data(), c->N * 8, hipMemcpyHostToDevice), "k tables");
+    rc = rc ? rc : check_hip(c, hipMemcpy(c->kshift, ks.data(), c->N * 16, hipMemcpyHostToDevice), "k tables");
+    rc = rc ? rc : check_hip(c, hipMemcpy(c->twiddle, tw.data(), c->N * 16, hipMemcpyHostToDevice), "twiddles");
+    // every field starts at zero, like a fresh LATfield2 Field
+    const size_t zero_sets[][2] = {
+        {(size_t)c->phi, (size_t)(s.n3 * 8)},        {(size_t)c->chi, (size_t)(s.n3 * 8)},
+        {(size_t)c->source, (size_t)(s.n3 * 8)},     {(size_t)c->Sij, (size_t)(6 * s.n3 * 8)},
+        {(size_t)c->Bi, (size_t)(3 * s.n3 * 8)},     {(size_t)c->scalarFT, (size_t)(s.nk * 16)},
+        {(size_t)c->SijFT, (size_t)(6 * s.nk * 16)}, {(size_t)c->BiFT, (size_t)(3 * s.nk * 16)},
+        {(size_t)c->fft_tmp, (size_t)(s.nk * 16)}};
+    for (auto &z : zero_sets)
+        if (!rc) rc = launch_fill_zero(c, (void *)z[0], z[1]);
+    if (!rc) rc = launch_refile(c);
+    if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "create sync");
+    if (rc) {
+        free_all(c);
+        if (c->own_stream) hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+GEV_EXPORT int gev_ctx_destroy(gev_ctx *c) {
+    if (!c) return GEV_EINVAL;
+    hipStreamSynchronize(c->stream);
+    for (auto &r : c->pending) {
+        hipEventDestroy(r.beg);
+        hipEventDestroy(r.end);
+    }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    free_all(c);
+    if (c->own_stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+GEV_EXPORT const char *gev_last_error(const gev_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+GEV_EXPORT int gev_ctx_synchronize(gev_ctx *c) {
+    if (!c) return GEV_EINVAL;
+    return check_hip(c, hipStreamSynchronize(c->stream), "synchronize");
+}
+
+// ---------------------------------------------------------------- data
+
+GEV_EXPORT int gev_upload_particles(gev_ctx *c, int64_t n, const double *pos, const double *q,
+                                    const int64_t *id, double mass) {
+    if (!c) return GEV_EINVAL;
+    if (n < 0 || n > c->cap) return fail(c, GEV_EINVAL, "particle count exceeds capacity");
+    if (n > 0 && (!pos || !q || !id)) return fail(c, GEV_EINVAL, "null particle buffer");
+    const double dx = 1.0 / (double)c->N;
+    std::vector<double> soa((size_t)6 * (size_t)(n > 0 ? n : 1));
+    for (int64_t p = 0; p < n; p++)
+        for (int i = 0; i < 3; i++) {
+            double x = pos[3 * p + i];
+            if (!(x >= 0.0 && x < 1.0) || (int)floor(x / dx) >= c->N)
+                return fail(c, GEV_EINVAL, "particle position outside [0,1)");
+            soa[(size_t)i * n + p] = x;
+            soa[(size_t)(3 + i) * n + p] = q[3 * p + i];
+        }
+    int rc = check_hip(c, hipStreamSynchronize(c->stream), "upload sync");
+    for (int i = 0; i < 3 && !rc && n > 0; i++) {
+        rc = check_hip(c, hipMemcpy(c->pos + (size_t)i * c->cap, soa.data() + (size_t)i * n, n * 8,
+                                    hipMemcpyHostToDevice), "upload pos");
+        if (!rc)
+            rc = check_hip(c, hipMemcpy(c->q + (size_t)i * c->cap, soa.data() + (size_t)(3 + i) * n,
+                                        n * 8, hipMemcpyHostToDevice), "upload q");
+    }
+    if (!rc && n > 0) rc = check_hip(c, hipMemcpy(c->id, id, n * 8, hipMemcpyHostToDevice), "upload id");
+    if (rc) return rc;
+    c->n = n;
+    c->pmass = mass;
+    return launch_refile(c);
+}
+
+GEV_EXPORT int gev_download_particles(gev_ctx *c, double *pos, double *q, int64_t *id) {
+    if (!c) return GEV_EINVAL;
+    const int64_t n = c->n;
+    if (n == 0) return 0;
+    std::vector<double> soa((size_t)6 * n);
+    int rc = check_hip(c, hipStreamSynchronize(c->stream), "download sync");
+    for (int i = 0; i < 3 && !rc; i++) {
+        rc = check_hip(c, hipMemcpy(soa.data() + (size_t)i * n, c->pos + (size_t)i * c->cap, n * 8,
+                                    hipMemcpyDeviceToHost), "download pos");
+        if (!rc)
+            rc = check_hip(c, hipMemcpy(soa.data() + (size_t)(3 + i) * n, c->q + (size_t)i * c->cap,
+                                        n * 8, hipMemcpyDeviceToHost), "download q");
+    }
+    if (!rc && id) rc = check_hip(c, hipMemcpy(id, c->id, n * 8, hipMemcpyDeviceToHost), "download id");
+    if (rc) return rc;
+    for (int64_t p = 0; p < n; p++)
+        for (int i = 0; i < 3; i++) {
+            if (pos) pos[3 * p + i] = soa[(size_t)i * n + p];
+            if (q) q[3 * p + i] = soa[(size_t)(3 + i) * n + p];
+        }
+    return 0;
+}
+
+GEV_EXPORT int64_t gev_particle_count(const gev_ctx *c) { return c ? c->n : -1; }
+
+GEV_EXPORT int gev_download_filing(gev_ctx *c, uint32_t *cell_count, uint32_t *particle_cell) {
+    if (!c) return GEV_EINVAL;
+    int rc = check_hip(c, hipStreamSynchronize(c->stream), "filing sync");
+    if (!rc && cell_count)
+        rc = check_hip(c, hipMemcpy(cell_count, c->count, n3(c) * 4, hipMemcpyDeviceToHost), "counts");
+    if (!rc && particle_cell && c->n > 0)
+        rc = check_hip(c, hipMemcpy(particle_cell, c->cell, c->n * 4, hipMemcpyDeviceToHost), "cells");
+    return rc;
+}
+
+static int field_desc(gev_ctx *c, gev_field f, void **ptr, int *ncomp, bool *complex_) {
+    *complex_ = false;
+    switch (f) {
+    case GEV_PHI: *ptr = c->phi; *ncomp = 1; return 0;
+    case GEV_CHI: *ptr = c->chi; *ncomp = 1; return 0;
+    case GEV_SOURCE: *ptr = c->source; *ncomp = 1; return 0;
+    case GEV_SIJ: *ptr = c->Sij; *ncomp = 6; return 0;
+    case GEV_BI: *ptr = c->Bi; *ncomp = 3; return 0;
+    case GEV_SCALARFT: *ptr = c->scalarFT; *ncomp = 1; *complex_ = true; return 0;
+    case GEV_SIJFT: *ptr = c->SijFT; *ncomp = 6; *complex_ = true; return 0;
+    case GEV_BIFT: *ptr = c->BiFT; *ncomp = 3; *complex_ = true; return 0;
+    }
+    return fail(c, GEV_EINVAL, "unknown field");
+}
+
+GEV_EXPORT int gev_upload_field(gev_ctx *c, gev_field f, const double *host) {
+    if (!c || !host) return GEV_EINVAL;
+    void *p;
+    int nc;
+    bool cx;
+    int rc = field_desc(c, f, &p, &nc, &cx);
+    if (rc) return rc;
+    rc = check_hip(c, hipStreamSynchronize(c->stream), "upload sync");
+    if (rc) return rc;
+    if (!cx) {
+        if ((int)f <= GEV_BI) c->zeroed[f] = false;
+        return check_hip(c, hipMemcpy(p, host, (size_t)nc * n3(c) * 8, hipMemcpyHostToDevice), "upload field");
+    }
+    const int H = c->N / 2 + 1;
+    return check_hip(c, hipMemcpy2D(p, (size_t)c->P * 16, host, (size_t)H * 16, (size_t)H * 16,
+                                    (size_t)nc * c->N * c->N, hipMemcpyHostToDevice),
+                     "upload k field");
+}
+
+GEV_EXPORT int gev_download_field(gev_ctx *c, gev_field f, double *host) {
+    if (!c || !host) return GEV_EINVAL;
+    void *p;
+    int nc;
+    bool cx;
+    int rc = field_desc(c, f, &p, &nc, &cx);
+    if (rc) return rc;
+    rc = check_hip(c, hipStreamSynchronize(c->stream), "download sync");
+    if (rc) return rc;
+    if (!cx)
+        return check_hip(c, hipMemcpy(host, p, (size_t)nc * n3(c) * 8, hipMemcpyDeviceToHost), "download field");
+    const int H = c->N / 2 + 1;
+    return check_hip(c, hipMemcpy2D(host, (size_t)H * 16, p, (size_t)c->P * 16, (size_t)H * 16,
+                                    (size_t)nc * c->N * c->N, hipMemcpyDeviceToHost),
+                     "download k field");
+}
+
+// ---------------------------------------------------------------- hot path
+
+GEV_EXPORT int gev_projection_init(gev_ctx *c, gev_field f) {
+    if (!c) return GEV_EINVAL;
+    void *p;
+    int nc;
+    bool cx;
+    int rc = field_desc(c, f, &p, &nc, &cx);
+    if (rc) return rc;
+    if (cx) return fail(c, GEV_EINVAL, "projection_init needs a real field");
+    rc = launch_fill_zero(c, p, (size_t)nc * n3(c) * 8);
+    if (!rc) c->zeroed[f] = true;
+    return rc;
+}
+
+GEV_EXPORT int gev_projection_T00_project(gev_ctx *c, double a, int use_phi, double coeff) {
+    if (!c) return GEV_EINVAL;
+    return launch_deposit(c, DEP_T00, a, use_phi, coeff);
+}
+
+GEV_EXPORT int gev_projection_Tij_project(gev_ctx *c, double a, int use_phi, double coeff) {
+    if (!c) return GEV_EINVAL;
+    return launch_deposit(c, DEP_TIJ, a, use_phi, coeff);
+}
+
+GEV_EXPORT int gev_projection_T00_Tij_project(gev_ctx *c, double a, double coeff) {
+    if (!c) return GEV_EINVAL;
+    return launch_deposit(c, DEP_T00_TIJ, a, 1, coeff);
+}
+
+GEV_EXPORT int gev_projection_T0i_project(gev_ctx *c, int use_phi, double coeff) {
+    if (!c) return GEV_EINVAL;
+    return launch_deposit(c, DEP_T0I, 1.0, use_phi, coeff);
+}
+
+GEV_EXPORT int gev_scalar_projection_cic_project(gev_ctx *c, double coeff) {
+    if (!c) return GEV_EINVAL;
+    return launch_deposit(c, DEP_CIC, 1.0, 0, coeff);
+}
+
+GEV_EXPORT int gev_field_sum(gev_ctx *c, gev_field f, double *sum) {
+    if (!c || !sum) return GEV_EINVAL;
+    void *p;
+    int nc;
+    bool cx;
+    int rc = field_desc(c, f, &p, &nc, &cx);
+    if (rc) return rc;
+    if (cx) return fail(c, GEV_EINVAL, "field_sum needs a real field");
+    return launch_field_sum(c, (const double *)p, n3(c), sum);
+}
+
+GEV_EXPORT int gev_prepare_ft_source_scalar(gev_ctx *c, double bgmodel, double coeff,
+                                            double coeff2, double coeff3) {
+    if (!c) return GEV_EINVAL;
+    c->zeroed[GEV_SOURCE] = false;
+    return launch_prepare_scalar(c, bgmodel, coeff, coeff2, coeff3);
+}
+
+GEV_EXPORT int gev_prepare_ft_source_tensor(gev_ctx *c, double coeff) {
+    if (!c) return GEV_EINVAL;
+    c->zeroed[GEV_SIJ] = false;
+    return launch_prepare_tensor(c, coeff);
+}
+
+GEV_EXPORT int gev_fft_execute(gev_ctx *c, gev_plan plan, int direction) {
+    if (!c) return GEV_EINVAL;
+    double *real;
+    double2 *k;
+    int nc;
+    gev_field rf;
+    switch (plan) {
+    case GEV_PLAN_SOURCE: real = c->source; k = c->scalarFT; nc = 1; rf = GEV_SOURCE; break;
+    case GEV_PLAN_PHI: real = c->phi; k = c->scalarFT; nc = 1; rf = GEV_PHI; break;
+    case GEV_PLAN_CHI: real = c->chi; k = c->scalarFT; nc = 1; rf = GEV_CHI; break;
+    case GEV_PLAN_SIJ: real = c->Sij; k = c->SijFT; nc = 6; rf = GEV_SIJ; break;
+    case GEV_PLAN_BI: real = c->Bi; k = c->BiFT; nc = 3; rf = GEV_BI; break;
+    default: return fail(c, GEV_EINVAL, "unknown FFT plan");
+    }
+    if (direction == GEV_FFT_FORWARD) return fft_forward(c, real, k, nc);
+    if (direction == GEV_FFT_BACKWARD) {
+        c->zeroed[rf] = false;
+        return fft_backward(c, k, real, nc);
+    }
+    return fail(c, GEV_EINVAL, "unknown FFT direction");
+}
+
+GEV_EXPORT int gev_solve_modified_poisson_ft(gev_ctx *c, double coeff, double modif) {
+    if (!c) return GEV_EINVAL;
+    return launch_poisson(c, coeff, modif);
+}
+
+GEV_EXPORT int gev_project_ft_scalar(gev_ctx *c, int add) {
+    if (!c) return GEV_EINVAL;
+    return launch_project_scalar(c, add);
+}
+
+GEV_EXPORT int gev_evolve_ft_vector(gev_ctx *c, double a2dtau) {
+    if (!c) return GEV_EINVAL;
+    return launch_evolve_vector(c, a2dtau);
+}
+
+GEV_EXPORT int gev_project_ft_scalar_evolve_vector(gev_ctx *c, double a2dtau) {
+    if (!c) return GEV_EINVAL;
+    return launch_project_scalar_evolve_vector(c, a2dtau);
+}
+
+GEV_EXPORT int gev_project_ft_vector(gev_ctx *c, double coeff, double modif) {
+    if (!c) return GEV_EINVAL;
+    return launch_project_vector(c, coeff, modif);
+}
+
+static bool valid_params(const double *p) { return p && p[0] != 0.0 && p[1] != 0.0; }
+
+GEV_EXPORT int gev_update_vel(gev_ctx *c, int method, double dtau, int nfield,
+                              const double params[2], double *maxvel) {
+    if (!c) return GEV_EINVAL;
+    if (method != GEV_Q_GR && method != GEV_Q_NEWTON) return fail(c, GEV_EINVAL, "unknown kick");
+    if (!valid_params(params)) return fail(c, GEV_EINVAL, "params must be {a, a^2 N}");
+    return launch_particle_update(c, true, method, dtau, nfield, params, false, 0, 0.0, 0, nullptr,
+                                  maxvel);
+}
+
+GEV_EXPORT int gev_move_particles(gev_ctx *c, int method, double dtau, int nfield,
+                                  const double params[2]) {
+    if (!c) return GEV_EINVAL;
+    if (method != GEV_POS_GR && method != GEV_POS_NEWTON) return fail(c, GEV_EINVAL, "unknown drift");
+    if (!valid_params(params)) return fail(c, GEV_EINVAL, "params must be {a, a^2 N}");
+    return launch_particle_update(c, false, 0, 0.0, 0, nullptr, true, method, dtau, nfield, params,
+                                  nullptr);
+}
+
+GEV_EXPORT int gev_kick_drift(gev_ctx *c, int qm, double dtk, int nfk, const double pk[2], int pm,
+                              double dtd, int nfd, const double pd[2], double *maxvel) {
+    if (!c) return GEV_EINVAL;
+    if ((qm != GEV_Q_GR && qm != GEV_Q_NEWTON) || (pm != GEV_POS_GR && pm != GEV_POS_NEWTON))
+        return fail(c, GEV_EINVAL, "unknown kick/drift method");
+    if (!valid_params(pk) || !valid_params(pd)) return fail(c, GEV_EINVAL, "params must be {a, a^2 N}");
+    return launch_particle_update(c, true, qm, dtk, nfk, pk, true, pm, dtd, nfd, pd, maxvel);
+}
+
+// ---------------------------------------------------------------- driver
+
+GEV_EXPORT double gev_hconf(double a, double fourpiG, const gev_cosmology *c) {
+    return hconf(a, fourpiG, c);
+}
+
+GEV_EXPORT void gev_rungekutta4bg(double *a, double fourpiG, const gev_cosmology *c, double dtau) {
+    rk4bg(*a, fourpiG, c, dtau);
+}
+
+GEV_EXPORT double gev_next_dtau(const gev_sim_params *p, int ngrid, double a) {
+    // main.cpp:292-295 / 869-872
+    const double dx = 1.0 / (double)ngrid;
+    if (p->Cf * dx < p->steplimit / hconf(a, p->fourpiG, &p->cosmo)) return p->Cf * dx;
+    return p->steplimit / hconf(a, p->fourpiG, &p->cosmo);
+}
+
+// One cycle of main.cpp:372-879 (one CDM species, no outputs).  Host scalars
+// are advanced exactly as the reference does; fields and particles never
+// leave the device.  T00hom, phi(k=0) and maxvel are left on the device and
+// read once by gev_run_cycles.
+static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st) {
+    const int N = c->N;
+    const double dx = 1.0 / (double)N;
+    const double fourpiG = p->fourpiG;
+    const gev_cosmology *cs = &p->cosmo;
+    double a = st->a;
+    int rc;
+
+    // projection_init(source), projection_init(Sij) + T00 and Tij deposits
+    // (main.cpp:378-450): the fused deposit stores every cell, so the
+    // zero fill is implied rather than executed.
+    c->zeroed[GEV_SOURCE] = true;
+    c->zeroed[GEV_SIJ] = true;
+    if (p->gr_flag > 0) {
+        rc = launch_deposit(c, DEP_T00_TIJ, a, 1, 1.0);
+    } else {
+        rc = launch_deposit(c, DEP_CIC, 1.0, 0, 1.0);
+        if (!rc) rc = launch_deposit(c, DEP_TIJ, a, 1, 1.0);
+    }
+    if (rc) return rc;
+    if (p->vector_flag == 1) {
+        c->zeroed[GEV_BI] = true;
+        rc = launch_deposit(c, DEP_T0I, 1.0, 1, 1.0);
+        if (rc) return rc;
+    }
+
+    if (p->gr_flag > 0) {
+        rc = launch_field_sum(c, c->source, n3(c), nullptr);  // T00hom * N^3 -> red_d[0]
+        if (rc) return rc;
+        if (st->dtau_old > 0.) {
+            const double H = hconf(a, fourpiG, cs);
+            rc = launch_prepare_scalar(c, cs->Omega_cdm + cs->Omega_b + 0.0,
+                                       3. * H * dx * dx / st->dtau_old, fourpiG * dx * dx / a,
+                                       3. * H * H * dx * dx);
+            if (!rc) rc = fft_forward(c, c->source, c->scalarFT, 1);
+            if (!rc) rc = launch_poisson(c, 1. / (dx * dx), 3. * hconf(a, fourpiG, cs) / st->dtau_old);
+            if (!rc) rc = fft_backward(c, c->scalarFT, c->phi, 1);
+            if (rc) return rc;
+        }
+    } else {
+        rc = fft_forward(c, c->source, c->scalarFT, 1);
+        if (!rc) rc = launch_poisson(c, fourpiG / a, 0.);
+        if (!rc) rc = fft_backward(c, c->scalarFT, c->phi, 1);
+        if (rc) return rc;
+    }
+    // phi(k=0) for the background record (main.cpp:533)
+    rc = check_hip(c, hipMemcpyAsync(c->red_d + 1, c->scalarFT, sizeof(double),
+                                     hipMemcpyDeviceToDevice, c->stream), "phi_k0");
+    if (rc) return rc;
+
+    rc = launch_prepare_tensor(c, 2. * fourpiG * dx * dx / a);
+    if (!rc) rc = fft_forward(c, c->Sij, c->SijFT, 6);
+    if (rc) return rc;
+    if (p->vector_flag == 1) {
+        rc = launch_project_scalar(c, 0);
+        if (!rc) rc = fft_backward(c, c->scalarFT, c->chi, 1);
+        if (!rc) rc = fft_forward(c, c->Bi, c->BiFT, 3);
+        if (!rc) rc = launch_project_vector(c, fourpiG * dx * dx, 0.);
+    } else {
+        rc = launch_project_scalar_evolve_vector(c, a * a * st->dtau_old);
+        if (!rc) rc = fft_backward(c, c->scalarFT, c->chi, 1);
+    }
+    if (rc) return rc;
+    if (p->gr_flag > 0) {
+        rc = fft_backward(c, c->BiFT, c->Bi, 3);
+        if (rc) return rc;
+    }
+    c->zeroed[GEV_PHI] = c->zeroed[GEV_CHI] = c->zeroed[GEV_BI] = false;
+
+    // particle update (main.cpp:766-822): kick with {a, a^2 N}, half RK4 step,
+    // drift with the advanced a, half RK4 step.  One fused particle pass.
+    double pk[2] = {a, a * a * N};
+    const double dtk = (st->dtau + st->dtau_old) / 2.;
+    const int nfk = p->gr_flag > 0 ? (1. / a < p->z_relax + 1. ? 3 : 2) : 1;
+    rk4bg(a, fourpiG, cs, 0.5 * st->dtau);
+    double pd[2] = {a, a * a * N};
+    const int nfd = p->gr_flag > 0 ? (1. / a < p->z_relax + 1. ? 3 : 0) : 0;
+    rc = launch_particle_update(c, true, p->gr_flag > 0 ? 0 : 1, dtk, nfk, pk, true,
+                                p->gr_flag > 0 ? 0 : 1, st->dtau, nfd, pd, nullptr);
+    if (rc) return rc;
+    rk4bg(a, fourpiG, cs, 0.5 * st->dtau);
+
+    st->tau += st->dtau;
+    st->dtau_old = st->dtau;
+    st->a = a;
+    st->dtau = gev_next_dtau(p, N, a);
+    st->cycle++;
+    return 0;
+}
+
+GEV_EXPORT int gev_run_cycles(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st,
+                              int ncycles) {
+    if (!c || !p || !st || ncycles < 0) return GEV_EINVAL;
+    if (p->gr_flag < 0 || p->gr_flag > 1 || p->vector_flag < 0 || p->vector_flag > 1)
+        return fail(c, GEV_EINVAL, "bad simulation flags");
+    if (!(st->a > 0.) || !(st->dtau > 0.)) return fail(c, GEV_EINVAL, "state needs a > 0, dtau > 0");
+    for (int i = 0; i < ncycles; i++) {
+        int rc = run_cycle(c, p, st);
+        if (rc) return rc;
+    }
+    if (ncycles > 0) {
+        double h[2];
+        unsigned long long vb;
+        int rc = check_hip(c, hipStreamSynchronize(c->stream), "cycle sync");
+        if (!rc) rc = check_hip(c, hipMemcpy(h, c->red_d, 2 * sizeof(double), hipMemcpyDeviceToHost), "cycle scalars");
+        if (!rc) rc = check_hip(c, hipMemcpy(&vb, c->red_u, sizeof(vb), hipMemcpyDeviceToHost), "cycle vmax");
+        if (rc) return rc;
+        double mv;
+        memcpy(&mv, &vb, sizeof(mv));
+        mv = sqrt(mv);
+        if (p->gr_flag > 0) mv /= sqrt(mv * mv + 1.0);
+        st->maxvel = mv;
+        st->T00hom = p->gr_flag > 0 ? h[0] / (double)n3(c) : 0.0;
+        st->phi_k0 = h[1];
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- timing
+
+GEV_EXPORT int gev_timing_enable(gev_ctx *c, int on) {
+    if (!c) return GEV_EINVAL;
+    if (!on) timers_collect(c);
+    c->timing = on != 0;
+    return 0;
+}
+
+GEV_EXPORT int gev_timing_reset(gev_ctx *c) {
+    if (!c) return GEV_EINVAL;
+    timers_collect(c);
+    c->acc.clear();
+    return 0;
+}
+
+GEV_EXPORT int gev_timing_report(gev_ctx *c, char *buf, size_t len) {
+    if (!c || !buf || len == 0) return GEV_EINVAL;
+    timers_collect(c);
+    std::string s;
+    char line[256];
+    for (auto &a : c->acc) {
+        snprintf(line, sizeof line, "%s %ld %.6f\n", a.name.c_str(), a.calls, a.ms);
+        s += line;
+    }
+    strncpy(buf, s.c_str(), len - 1);
+    buf[len - 1] = 0;
+    return 0;
+}

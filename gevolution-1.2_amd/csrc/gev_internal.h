@@ -1,0 +1,126 @@
+// gev_internal.h -- context layout and helpers shared by the HIP sources of
+// libgevpm.so.  Not part of the public ABI (include/gev_pm.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gev_pm.h"
+
+namespace gev {
+
+// One complex k-space row holds kx in [0, N/2]; rows are padded to a multiple
+// of 8 complexes so that every row starts on a 128-byte boundary (FP64).
+inline int kpitch(int N) { return ((N / 2 + 1) + 7) / 8 * 8; }
+
+struct TimerRec {
+    const char *name;
+    hipEvent_t beg, end;
+};
+
+struct TimerAcc {
+    std::string name;
+    long calls = 0;
+    double ms = 0.0;
+};
+
+}  // namespace gev
+
+struct gev_ctx {
+    int N = 0;
+    int P = 0;            // k-space row pitch (complex elements)
+    int64_t cap = 0;      // particle capacity
+    int64_t n = 0;        // particles resident
+    double pmass = 0.0;   // species mass (part_simple_info)
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    // real fields, N^3 doubles per component
+    double *phi = nullptr, *chi = nullptr, *source = nullptr, *Sij = nullptr, *Bi = nullptr;
+    // complex fields, P*N*N complexes per component
+    double2 *scalarFT = nullptr, *SijFT = nullptr, *BiFT = nullptr, *fft_tmp = nullptr;
+    // FFT twiddles exp(-2 pi i t/N), t < N; k-space tables
+    double2 *twiddle = nullptr;
+    double *gridk2 = nullptr;   // (2N sin(pi n/N))^2
+    double2 *kshift = nullptr;  // 2N sin(pi n/N) exp(-i pi n/N)
+
+    // particles, structure of arrays
+    double *pos = nullptr;  // [3][cap]
+    double *q = nullptr;    // [3][cap]
+    int64_t *id = nullptr;
+    uint32_t *cell = nullptr;   // filed cell of every particle
+    uint32_t *slot = nullptr;   // rank inside the cell from the counting pass
+    uint32_t *perm = nullptr;   // particle indices grouped by cell, ascending ID
+    uint32_t *count = nullptr;  // N^3 per-cell counts
+    uint32_t *start = nullptr;  // N^3 + 1 CSR offsets
+    uint32_t *scan_part = nullptr;  // scan block partials
+    int64_t scan_nblk = 0;
+
+    // reductions
+    double *red_d = nullptr;              // device double scratch (sums)
+    unsigned long long *red_u = nullptr;  // device max-as-bits scratch
+    double *host_pinned = nullptr;        // pinned readback slot
+
+    // state: field known to be all zero (projection_init), so deposits store
+    bool zeroed[5] = {false, false, false, false, false};
+
+    std::string err;
+
+    // timing
+    bool timing = false;
+    std::vector<gev::TimerRec> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<gev::TimerAcc> acc;
+};
+
+namespace gev {
+
+// sets ctx->err, returns code
+int fail(gev_ctx *ctx, int code, const std::string &msg);
+int check_hip(gev_ctx *ctx, hipError_t e, const char *what);
+
+// launch bracketing for the -DBENCHMARK style timers
+void timer_begin(gev_ctx *ctx, const char *name);
+void timer_end(gev_ctx *ctx);
+
+struct ScopedTimer {
+    gev_ctx *c;
+    ScopedTimer(gev_ctx *ctx, const char *name) : c(ctx) { timer_begin(ctx, name); }
+    ~ScopedTimer() { timer_end(c); }
+};
+
+inline int64_t n3(const gev_ctx *c) { return (int64_t)c->N * c->N * c->N; }
+inline int64_t nk(const gev_ctx *c) { return (int64_t)c->P * c->N * c->N; }
+
+inline unsigned grid_for(int64_t work, int block) {
+    return (unsigned)((work + block - 1) / block);
+}
+
+// kernels implemented in the .hip files, called by api.hip
+int fft_forward(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp);
+int fft_backward(gev_ctx *ctx, const double2 *kfield, double *real, int ncomp);
+
+int launch_poisson(gev_ctx *ctx, double coeff, double modif);
+int launch_project_scalar(gev_ctx *ctx, int add);
+int launch_evolve_vector(gev_ctx *ctx, double a2dtau);
+int launch_project_scalar_evolve_vector(gev_ctx *ctx, double a2dtau);
+int launch_project_vector(gev_ctx *ctx, double coeff, double modif);
+
+int launch_prepare_scalar(gev_ctx *ctx, double bg, double c1, double c2, double c3);
+int launch_prepare_tensor(gev_ctx *ctx, double coeff);
+int launch_field_sum(gev_ctx *ctx, const double *f, int64_t count, double *out_host);
+int launch_fill_zero(gev_ctx *ctx, void *ptr, size_t bytes);
+
+enum DepositKind { DEP_T00 = 1, DEP_TIJ = 2, DEP_T00_TIJ = 3, DEP_T0I = 4, DEP_CIC = 5 };
+int launch_deposit(gev_ctx *ctx, int kind, double a, int use_phi, double coeff);
+
+int launch_refile(gev_ctx *ctx);  // cells from positions, then CSR
+int launch_particle_update(gev_ctx *ctx, bool kick, int qm, double dtk, int nfk, const double pk[2],
+                           bool drift, int pm, double dtd, int nfd, const double pd[2],
+                           double *maxvel);
+
+}  // namespace gev

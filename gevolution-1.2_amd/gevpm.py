@@ -1,0 +1,341 @@
+"""Python binding of libgevpm.so (include/gev_pm.h) through ctypes.
+
+`Box` mirrors the call surface that gevolution's main loop drives
+(main.cpp:372-879): the method names are the reference's function names and
+the arguments keep their meaning.  Host arrays are numpy; device memory is
+owned by the library.  Importing this module never falls back to anything:
+if the HIP library is missing or fails to load, an ImportError/OSError is
+raised.
+
+Array conventions (shared with the oracle used by the tests):
+  real fields  float64 (N, N, N) indexed [z, y, x]; Sij (6, N, N, N) in the
+  order 00,01,02,11,12,22; Bi (3, N, N, N);
+  k fields     complex128 (N, N, N//2+1) indexed [kz, ky, kx] (numpy rfftn);
+  particles    pos, q float64 (n, 3); id int64 (n,).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgevpm.so")
+
+_d, _i, _i64, _p, _sz = ctypes.c_double, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
+
+PHI, CHI, SOURCE, SIJ, BI, SCALARFT, SIJFT, BIFT = range(8)
+PLAN_SOURCE, PLAN_PHI, PLAN_CHI, PLAN_SIJ, PLAN_BI = range(5)
+FFT_FORWARD, FFT_BACKWARD = 0, 1
+Q_GR, Q_NEWTON = 0, 1
+POS_GR, POS_NEWTON = 0, 1
+
+_NCOMP = {PHI: 1, CHI: 1, SOURCE: 1, SIJ: 6, BI: 3, SCALARFT: 1, SIJFT: 6, BIFT: 3}
+
+EXPORTED = [
+    "gev_ctx_footprint_bytes", "gev_ctx_create", "gev_ctx_destroy", "gev_last_error",
+    "gev_ctx_synchronize", "gev_upload_particles", "gev_download_particles",
+    "gev_particle_count", "gev_download_filing", "gev_upload_field", "gev_download_field",
+    "gev_projection_init", "gev_projection_T00_project", "gev_projection_Tij_project",
+    "gev_projection_T00_Tij_project", "gev_projection_T0i_project",
+    "gev_scalar_projection_cic_project", "gev_field_sum", "gev_prepare_ft_source_scalar",
+    "gev_prepare_ft_source_tensor", "gev_fft_execute", "gev_solve_modified_poisson_ft",
+    "gev_project_ft_scalar", "gev_evolve_ft_vector", "gev_project_ft_scalar_evolve_vector",
+    "gev_project_ft_vector", "gev_update_vel", "gev_move_particles", "gev_kick_drift",
+    "gev_hconf", "gev_rungekutta4bg", "gev_next_dtau", "gev_run_cycles", "gev_timing_enable",
+    "gev_timing_reset", "gev_timing_report",
+]
+
+
+class BoxDesc(ctypes.Structure):
+    _fields_ = [("ngrid", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("npart", ctypes.c_int64), ("stream", _p)]
+
+
+class Cosmology(ctypes.Structure):
+    _fields_ = [("Omega_cdm", _d), ("Omega_b", _d), ("Omega_m", _d), ("Omega_Lambda", _d),
+                ("Omega_rad", _d), ("Omega_fld", _d), ("w0_fld", _d), ("wa_fld", _d), ("h", _d)]
+
+
+class SimParams(ctypes.Structure):
+    _fields_ = [("gr_flag", ctypes.c_int32), ("vector_flag", ctypes.c_int32), ("Cf", _d),
+                ("steplimit", _d), ("z_relax", _d), ("fourpiG", _d), ("cosmo", Cosmology)]
+
+
+class CycleState(ctypes.Structure):
+    _fields_ = [("a", _d), ("tau", _d), ("dtau", _d), ("dtau_old", _d), ("maxvel", _d),
+                ("T00hom", _d), ("phi_k0", _d), ("cycle", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libgevpm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libgevpm.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.gev_ctx_footprint_bytes.restype = _i64
+        L.gev_ctx_footprint_bytes.argtypes = [_p]
+        L.gev_ctx_create.argtypes = [_p, ctypes.POINTER(_p)]
+        L.gev_ctx_destroy.argtypes = [_p]
+        L.gev_last_error.restype = ctypes.c_char_p
+        L.gev_last_error.argtypes = [_p]
+        L.gev_ctx_synchronize.argtypes = [_p]
+        L.gev_upload_particles.argtypes = [_p, _i64, _p, _p, _p, _d]
+        L.gev_download_particles.argtypes = [_p, _p, _p, _p]
+        L.gev_particle_count.restype = _i64
+        L.gev_particle_count.argtypes = [_p]
+        L.gev_download_filing.argtypes = [_p, _p, _p]
+        L.gev_upload_field.argtypes = [_p, _i, _p]
+        L.gev_download_field.argtypes = [_p, _i, _p]
+        L.gev_projection_init.argtypes = [_p, _i]
+        L.gev_projection_T00_project.argtypes = [_p, _d, _i, _d]
+        L.gev_projection_Tij_project.argtypes = [_p, _d, _i, _d]
+        L.gev_projection_T00_Tij_project.argtypes = [_p, _d, _d]
+        L.gev_projection_T0i_project.argtypes = [_p, _i, _d]
+        L.gev_scalar_projection_cic_project.argtypes = [_p, _d]
+        L.gev_field_sum.argtypes = [_p, _i, ctypes.POINTER(_d)]
+        L.gev_prepare_ft_source_scalar.argtypes = [_p, _d, _d, _d, _d]
+        L.gev_prepare_ft_source_tensor.argtypes = [_p, _d]
+        L.gev_fft_execute.argtypes = [_p, _i, _i]
+        L.gev_solve_modified_poisson_ft.argtypes = [_p, _d, _d]
+        L.gev_project_ft_scalar.argtypes = [_p, _i]
+        L.gev_evolve_ft_vector.argtypes = [_p, _d]
+        L.gev_project_ft_scalar_evolve_vector.argtypes = [_p, _d]
+        L.gev_project_ft_vector.argtypes = [_p, _d, _d]
+        L.gev_update_vel.argtypes = [_p, _i, _d, _i, _p, ctypes.POINTER(_d)]
+        L.gev_move_particles.argtypes = [_p, _i, _d, _i, _p]
+        L.gev_kick_drift.argtypes = [_p, _i, _d, _i, _p, _i, _d, _i, _p, ctypes.POINTER(_d)]
+        L.gev_hconf.restype = _d
+        L.gev_hconf.argtypes = [_d, _d, _p]
+        L.gev_rungekutta4bg.restype = None
+        L.gev_rungekutta4bg.argtypes = [ctypes.POINTER(_d), _d, _p, _d]
+        L.gev_next_dtau.restype = _d
+        L.gev_next_dtau.argtypes = [_p, _i, _d]
+        L.gev_run_cycles.argtypes = [_p, _p, _p, _i]
+        L.gev_timing_enable.argtypes = [_p, _i]
+        L.gev_timing_reset.argtypes = [_p]
+        L.gev_timing_report.argtypes = [_p, ctypes.c_char_p, _sz]
+        _lib = L
+    return _lib
+
+
+class GevError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+def make_cosmology(Omega_cdm=0.26, Omega_b=0.05, h=0.67, Omega_rad=0.0):
+    om = Omega_cdm + Omega_b
+    return Cosmology(Omega_cdm, Omega_b, om, 1.0 - om - Omega_rad, Omega_rad, 0.0, -1.0, 0.0, h)
+
+
+def hconf(a, fourpiG, cosmo):
+    return lib().gev_hconf(a, fourpiG, ctypes.byref(cosmo))
+
+
+def rungekutta4bg(a, fourpiG, cosmo, dtau):
+    x = _d(a)
+    lib().gev_rungekutta4bg(ctypes.byref(x), fourpiG, ctypes.byref(cosmo), dtau)
+    return x.value
+
+
+def footprint_bytes(ngrid, npart):
+    d = BoxDesc(ngrid, 0, npart, None)
+    return lib().gev_ctx_footprint_bytes(ctypes.byref(d))
+
+
+class Box:
+    """One simulation box resident on the current GPU (a gev_ctx)."""
+
+    def __init__(self, ngrid, npart, stream=None):
+        self.N = int(ngrid)
+        self.cap = int(npart)
+        self._c = _p()
+        d = BoxDesc(self.N, 0, self.cap, stream)
+        rc = lib().gev_ctx_create(ctypes.byref(d), ctypes.byref(self._c))
+        if rc != 0:
+            raise GevError("gev_ctx_create(N=%d, npart=%d) failed with %d" % (self.N, self.cap, rc))
+
+    # -- plumbing
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise GevError("%s failed (%d): %s" % (what, rc, lib().gev_last_error(self._c).decode()))
+
+    def close(self):
+        if self._c:
+            lib().gev_ctx_destroy(self._c)
+            self._c = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        self._chk(lib().gev_ctx_synchronize(self._c), "synchronize")
+
+    # -- data movement
+    def upload_particles(self, pos, q, ids, mass):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        n = pos.shape[0]
+        if q.shape != (n, 3) or pos.shape != (n, 3) or ids.shape != (n,):
+            raise ValueError("pos/q must be (n,3) and id (n,)")
+        self._chk(lib().gev_upload_particles(self._c, n, _ptr(pos), _ptr(q), _ptr(ids), mass),
+                  "upload_particles")
+
+    @property
+    def n(self):
+        return lib().gev_particle_count(self._c)
+
+    def download_particles(self):
+        n = self.n
+        pos = np.zeros((n, 3))
+        q = np.zeros((n, 3))
+        ids = np.zeros(n, dtype=np.int64)
+        if n:
+            self._chk(lib().gev_download_particles(self._c, _ptr(pos), _ptr(q), _ptr(ids)),
+                      "download_particles")
+        return pos, q, ids
+
+    def download_filing(self):
+        counts = np.zeros(self.N ** 3, dtype=np.uint32)
+        cells = np.zeros(max(self.n, 1), dtype=np.uint32)
+        self._chk(lib().gev_download_filing(self._c, _ptr(counts), _ptr(cells)), "download_filing")
+        return counts, cells[:self.n]
+
+    def _shape(self, f):
+        N, nc = self.N, _NCOMP[f]
+        if f >= SCALARFT:
+            shp, dt = (N, N, N // 2 + 1), np.complex128
+        else:
+            shp, dt = (N, N, N), np.float64
+        return ((nc,) + shp if nc > 1 else shp), dt
+
+    def upload_field(self, f, arr):
+        shp, dt = self._shape(f)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if a.shape != shp:
+            raise ValueError("field %d expects shape %s, got %s" % (f, shp, a.shape))
+        self._chk(lib().gev_upload_field(self._c, f, _ptr(a)), "upload_field")
+
+    def download_field(self, f):
+        shp, dt = self._shape(f)
+        a = np.zeros(shp, dtype=dt)
+        self._chk(lib().gev_download_field(self._c, f, _ptr(a)), "download_field")
+        return a
+
+    # -- the reference call surface
+    def projection_init(self, f):
+        self._chk(lib().gev_projection_init(self._c, f), "projection_init")
+
+    def projection_T00_project(self, a=1.0, use_phi=True, coeff=1.0):
+        self._chk(lib().gev_projection_T00_project(self._c, a, int(use_phi), coeff), "projection_T00_project")
+
+    def projection_Tij_project(self, a=1.0, use_phi=True, coeff=1.0):
+        self._chk(lib().gev_projection_Tij_project(self._c, a, int(use_phi), coeff), "projection_Tij_project")
+
+    def projection_T00_Tij_project(self, a=1.0, coeff=1.0):
+        self._chk(lib().gev_projection_T00_Tij_project(self._c, a, coeff), "projection_T00_Tij_project")
+
+    def projection_T0i_project(self, use_phi=True, coeff=1.0):
+        self._chk(lib().gev_projection_T0i_project(self._c, int(use_phi), coeff), "projection_T0i_project")
+
+    def scalarProjectionCIC_project(self, coeff=1.0):
+        self._chk(lib().gev_scalar_projection_cic_project(self._c, coeff), "scalarProjectionCIC_project")
+
+    def field_sum(self, f):
+        s = _d()
+        self._chk(lib().gev_field_sum(self._c, f, ctypes.byref(s)), "field_sum")
+        return s.value
+
+    def prepareFTsource_scalar(self, bgmodel, coeff, coeff2, coeff3):
+        self._chk(lib().gev_prepare_ft_source_scalar(self._c, bgmodel, coeff, coeff2, coeff3),
+                  "prepareFTsource(scalar)")
+
+    def prepareFTsource_tensor(self, coeff):
+        self._chk(lib().gev_prepare_ft_source_tensor(self._c, coeff), "prepareFTsource(tensor)")
+
+    def fft(self, plan, direction):
+        self._chk(lib().gev_fft_execute(self._c, plan, direction), "fft")
+
+    def solveModifiedPoissonFT(self, coeff, modif=0.0):
+        self._chk(lib().gev_solve_modified_poisson_ft(self._c, coeff, modif), "solveModifiedPoissonFT")
+
+    def projectFTscalar(self, add=0):
+        self._chk(lib().gev_project_ft_scalar(self._c, add), "projectFTscalar")
+
+    def evolveFTvector(self, a2dtau):
+        self._chk(lib().gev_evolve_ft_vector(self._c, a2dtau), "evolveFTvector")
+
+    def projectFTscalar_evolveFTvector(self, a2dtau):
+        self._chk(lib().gev_project_ft_scalar_evolve_vector(self._c, a2dtau),
+                  "projectFTscalar+evolveFTvector")
+
+    def projectFTvector(self, coeff=1.0, modif=0.0):
+        self._chk(lib().gev_project_ft_vector(self._c, coeff, modif), "projectFTvector")
+
+    def updateVel(self, method, dtau, nfield, params, want_maxvel=True):
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        mv = _d()
+        self._chk(lib().gev_update_vel(self._c, method, dtau, nfield, _ptr(prm),
+                                       ctypes.byref(mv) if want_maxvel else None), "updateVel")
+        return mv.value if want_maxvel else None
+
+    def moveParticles(self, method, dtau, nfield, params):
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(lib().gev_move_particles(self._c, method, dtau, nfield, _ptr(prm)), "moveParticles")
+
+    def kick_drift(self, q_method, dtau_kick, nfield_kick, params_kick, pos_method, dtau_drift,
+                   nfield_drift, params_drift, want_maxvel=True):
+        pk = np.ascontiguousarray(params_kick, dtype=np.float64)
+        pd = np.ascontiguousarray(params_drift, dtype=np.float64)
+        mv = _d()
+        self._chk(lib().gev_kick_drift(self._c, q_method, dtau_kick, nfield_kick, _ptr(pk),
+                                       pos_method, dtau_drift, nfield_drift, _ptr(pd),
+                                       ctypes.byref(mv) if want_maxvel else None), "kick_drift")
+        return mv.value if want_maxvel else None
+
+    def run_cycles(self, sim, state, ncycles):
+        self._chk(lib().gev_run_cycles(self._c, ctypes.byref(sim), ctypes.byref(state), ncycles),
+                  "run_cycles")
+        return state
+
+    # -- timing
+    def timing(self, on=True):
+        self._chk(lib().gev_timing_enable(self._c, int(on)), "timing_enable")
+
+    def timing_reset(self):
+        self._chk(lib().gev_timing_reset(self._c), "timing_reset")
+
+    def timing_report(self):
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._chk(lib().gev_timing_report(self._c, buf, len(buf)), "timing_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, calls, ms = line.split()
+            out[name] = (int(calls), float(ms))
+        return out
+
+
+def next_dtau(sim, ngrid, a):
+    return lib().gev_next_dtau(ctypes.byref(sim), ngrid, a)

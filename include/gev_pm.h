@@ -1,0 +1,226 @@
+/*
+ * gev_pm.h -- C ABI of the MI355X particle-mesh library (libgevpm.so).
+ *
+ * One simulation box (fields + one particle species) lives whole in one GPU's
+ * HBM inside a gev_ctx.  Each hot-path entry point replaces one call of
+ * gevolution 1.2's main loop (main.cpp:372-879) and keeps that call's
+ * argument meaning; the reference function it replaces is cited beside it
+ * (file:line in the gevolution-1.2 source tree).  LATfield2's halo folds,
+ * halo updates and particle migration (projection_*_comm, updateHalo, the MPI
+ * part of moveParticles) have no entry point: fields are periodic arrays and
+ * indices wrap.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative GEV_E* code; the
+ *     message is available from gev_last_error(ctx).  Nothing calls exit().
+ *   - Calls are asynchronous on the context's HIP stream unless stated;
+ *     functions returning host values synchronise that stream.
+ *   - Host buffers are borrowed for the duration of the call only.
+ *   - Real fields: N^3 doubles, index x + N*(y + N*z).  Tensor fields: 6
+ *     components in the order 00,01,02,11,12,22.  Vector fields: 3.
+ *     k-space fields on the host: N*N*(N/2+1) complex (re,im) doubles, index
+ *     kx + (N/2+1)*(ky + N*kz) (numpy rfftn layout of a (z,y,x) array).
+ *   - Particle positions are in box units [0,1); q is the canonical momentum
+ *     per unit mass (gevolution.hpp:543-546).
+ *   - A context is not thread-safe; distinct contexts may be used from
+ *     distinct threads.
+ */
+#ifndef GEV_PM_H
+#define GEV_PM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gev_ctx gev_ctx;
+
+enum {
+    GEV_OK = 0,
+    GEV_EINVAL = -1,   /* bad argument */
+    GEV_ENOMEM = -2,   /* device allocation failed */
+    GEV_EHIP = -3,     /* HIP runtime error */
+    GEV_ESTATE = -4    /* call not valid in the current context state */
+};
+
+/* fields (LATfield2 Field<> objects of main.cpp:226-246) */
+typedef enum {
+    GEV_PHI = 0,      /* phi       Field<Real>, 1 comp   */
+    GEV_CHI = 1,      /* chi       Field<Real>, 1 comp   */
+    GEV_SOURCE = 2,   /* source    Field<Real>, 1 comp   */
+    GEV_SIJ = 3,      /* Sij       Field<Real>, 6 comps  */
+    GEV_BI = 4,       /* Bi        Field<Real>, 3 comps  */
+    GEV_SCALARFT = 5, /* scalarFT  Field<Cplx>, 1 comp   */
+    GEV_SIJFT = 6,    /* SijFT     Field<Cplx>, 6 comps  */
+    GEV_BIFT = 7      /* BiFT      Field<Cplx>, 3 comps  */
+} gev_field;
+
+/* FFT plans, main.cpp:238-246 (PlanFFT<Cplx> plan_x(&real, &kfield)) */
+typedef enum {
+    GEV_PLAN_SOURCE = 0, /* source <-> scalarFT */
+    GEV_PLAN_PHI = 1,    /* phi    <-> scalarFT */
+    GEV_PLAN_CHI = 2,    /* chi    <-> scalarFT */
+    GEV_PLAN_SIJ = 3,    /* Sij    <-> SijFT    */
+    GEV_PLAN_BI = 4      /* Bi     <-> BiFT     */
+} gev_plan;
+
+enum { GEV_FFT_FORWARD = 0, GEV_FFT_BACKWARD = 1 };
+
+/* per-particle callbacks of updateVel / moveParticles (gevolution.hpp:570-903)
+   are selected by enum; no host function pointers cross the ABI */
+enum { GEV_Q_GR = 0, GEV_Q_NEWTON = 1 };     /* update_q, update_q_Newton     */
+enum { GEV_POS_GR = 0, GEV_POS_NEWTON = 1 }; /* update_pos, update_pos_Newton */
+
+typedef struct {
+    int32_t ngrid;      /* N: lattice points per dimension (2^a 3^b, 8..1024)  */
+    int32_t reserved0;
+    int64_t npart;      /* particle capacity                                   */
+    void *stream;       /* hipStream_t to run on; NULL = a new stream          */
+} gev_box_desc;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Device bytes a context for `desc` allocates (used for admission). */
+int64_t gev_ctx_footprint_bytes(const gev_box_desc *desc);
+/* Replaces the Field::initialize / PlanFFT construction block of
+   main.cpp:209-246 for one box. */
+int gev_ctx_create(const gev_box_desc *desc, gev_ctx **out);
+int gev_ctx_destroy(gev_ctx *ctx);
+const char *gev_last_error(const gev_ctx *ctx);
+/* Blocks until all work queued on the context stream has finished. */
+int gev_ctx_synchronize(gev_ctx *ctx);
+
+/* ---- data movement (IC upload, parity checks) --------------------------- */
+
+/* n particles, AoS as part_simple: pos[n][3], q[n][3], id[n]; mass is the
+   species mass of part_simple_info (ic_basic.hpp:1987).  Files the particles
+   into cells (as Particles::addParticle_global does). */
+int gev_upload_particles(gev_ctx *ctx, int64_t n, const double *pos, const double *q,
+                         const int64_t *id, double mass);
+/* Copies particles back in storage order (not sorted by ID). */
+int gev_download_particles(gev_ctx *ctx, double *pos, double *q, int64_t *id);
+int64_t gev_particle_count(const gev_ctx *ctx);
+/* Per-cell particle counts (N^3 uint32) and the cell index of every particle
+   in storage order (n uint32) as filed by the last moveParticles/upload. */
+int gev_download_filing(gev_ctx *ctx, uint32_t *cell_count, uint32_t *particle_cell);
+
+int gev_upload_field(gev_ctx *ctx, gev_field f, const double *host);
+int gev_download_field(gev_ctx *ctx, gev_field f, double *host);
+
+/* ---- hot path: one entry per reference call ----------------------------- */
+
+/* projection_init (LATfield2; main.cpp:378,438): zero a real field. */
+int gev_projection_init(gev_ctx *ctx, gev_field f);
+
+/* projection_T00_project, gevolution.hpp:926-1022 (main.cpp:385):
+   source += T00 deposit.  use_phi = 0 passes phi = NULL. */
+int gev_projection_T00_project(gev_ctx *ctx, double a, int use_phi, double coeff);
+/* projection_Tij_project, gevolution.hpp:1172-1297 (main.cpp:439): Sij += Tij. */
+int gev_projection_Tij_project(gev_ctx *ctx, double a, int use_phi, double coeff);
+/* projection_T00_project + projection_Tij_project fused into one particle
+   pass (main.cpp:385 and 439 with the same a, phi and coeff). */
+int gev_projection_T00_Tij_project(gev_ctx *ctx, double a, double coeff);
+/* projection_T0i_project, gevolution.hpp:1045-1147 (main.cpp:427): Bi += T0i. */
+int gev_projection_T0i_project(gev_ctx *ctx, int use_phi, double coeff);
+/* scalarProjectionCIC_project (LATfield2; main.cpp:402): source += CIC of
+   coeff*mass/dx^3. */
+int gev_scalar_projection_cic_project(gev_ctx *ctx, double coeff);
+
+/* Sum of a real field's first component (main.cpp:459-462, T00hom*N^3);
+   synchronous. */
+int gev_field_sum(gev_ctx *ctx, gev_field f, double *sum);
+
+/* prepareFTsource (2), gevolution.hpp:169-192 (main.cpp:472):
+   source <- f(phi, chi, source). */
+int gev_prepare_ft_source_scalar(gev_ctx *ctx, double bgmodel, double coeff, double coeff2,
+                                 double coeff3);
+/* prepareFTsource (1), gevolution.hpp:56-147 (main.cpp:539): Sij <- f(phi, Sij). */
+int gev_prepare_ft_source_tensor(gev_ctx *ctx, double coeff);
+
+/* PlanFFT<Cplx>::execute (LATfield2 over FFTW3; main.cpp:477,488,544,563,575,593).
+   Unnormalised r2c forward / c2r backward; backward preserves the k field. */
+int gev_fft_execute(gev_ctx *ctx, gev_plan plan, int direction);
+
+/* solveModifiedPoissonFT(scalarFT, scalarFT, coeff, modif),
+   gevolution.hpp:501-535 (main.cpp:483, 506). */
+int gev_solve_modified_poisson_ft(gev_ctx *ctx, double coeff, double modif);
+/* projectFTscalar(SijFT, scalarFT, add), gevolution.hpp:211-265 (main.cpp:558). */
+int gev_project_ft_scalar(gev_ctx *ctx, int add);
+/* evolveFTvector(SijFT, BiFT, a2dtau), gevolution.hpp:284-330 (main.cpp:586). */
+int gev_evolve_ft_vector(gev_ctx *ctx, double a2dtau);
+/* projectFTscalar(SijFT, scalarFT, 0) + evolveFTvector(SijFT, BiFT, a2dtau)
+   fused into one pass over SijFT (main.cpp:558 and 586). */
+int gev_project_ft_scalar_evolve_vector(gev_ctx *ctx, double a2dtau);
+/* projectFTvector(BiFT, BiFT, coeff, modif), gevolution.hpp:350-392
+   (main.cpp:580; ic_basic.hpp:2238). */
+int gev_project_ft_vector(gev_ctx *ctx, double coeff, double modif);
+
+/* Particles::updateVel(update_q | update_q_Newton, dtau, {phi,chi,Bi}, nfield,
+   params) (LATfield2 driver; gevolution.hpp:570-776; main.cpp:771-781).
+   params = {a, a^2 N}.  *maxvel (may be NULL; synchronous if not) receives
+   sqrt(max over particles of the callback's return). */
+int gev_update_vel(gev_ctx *ctx, int method, double dtau, int nfield, const double params[2],
+                   double *maxvel);
+/* Particles::moveParticles(update_pos | update_pos_Newton, dtau, ..., nfield,
+   params) (gevolution.hpp:810-903; main.cpp:794-804) including the periodic
+   wrap and re-filing into cells. */
+int gev_move_particles(gev_ctx *ctx, int method, double dtau, int nfield,
+                       const double params[2]);
+/* updateVel followed by moveParticles in one particle pass (main.cpp:771-800).
+   The host advances a by rungekutta4bg between the two calls in the reference;
+   here the caller passes both parameter sets.  *maxvel as gev_update_vel. */
+int gev_kick_drift(gev_ctx *ctx, int q_method, double dtau_kick, int nfield_kick,
+                   const double params_kick[2], int pos_method, double dtau_drift,
+                   int nfield_drift, const double params_drift[2], double *maxvel);
+
+/* ---- cycle driver (main.cpp:372-879 for one CDM species) ----------------- */
+
+typedef struct {
+    double Omega_cdm, Omega_b, Omega_m, Omega_Lambda, Omega_rad;
+    double Omega_fld, w0_fld, wa_fld;
+    double h;
+} gev_cosmology;
+
+typedef struct {
+    int32_t gr_flag;     /* 1 = GR, 0 = Newton                       */
+    int32_t vector_flag; /* 0 = parabolic, 1 = elliptic              */
+    double Cf;           /* Courant factor                           */
+    double steplimit;    /* time step limit (fraction of Hubble time)*/
+    double z_relax;      /* relaxation redshift (nfield switch)      */
+    double fourpiG;      /* 1.5 boxsize^2 / c^2                      */
+    gev_cosmology cosmo;
+} gev_sim_params;
+
+typedef struct {
+    double a, tau, dtau, dtau_old, maxvel, T00hom, phi_k0;
+    int32_t cycle;
+    int32_t reserved0;
+} gev_cycle_state;
+
+/* Conformal Hubble rate, background.hpp:137-140 (no ncdm). */
+double gev_hconf(double a, double fourpiG, const gev_cosmology *c);
+/* rungekutta4bg, background.hpp:167-177. */
+void gev_rungekutta4bg(double *a, double fourpiG, const gev_cosmology *c, double dtau);
+/* Time step rule of main.cpp:292-295 / 869-872. */
+double gev_next_dtau(const gev_sim_params *p, int ngrid, double a);
+
+/* Runs `ncycles` cycles of the main loop (no outputs) and advances *st as
+   main.cpp:825-874 does.  Synchronous at the end (reads T00hom, maxvel). */
+int gev_run_cycles(gev_ctx *ctx, const gev_sim_params *p, gev_cycle_state *st, int ncycles);
+
+/* ---- timing (the -DBENCHMARK categories of main.cpp:71-88) --------------- */
+
+/* When enabled, every kernel launch is bracketed by HIP events on the context
+   stream and its duration is accumulated per kernel name. */
+int gev_timing_enable(gev_ctx *ctx, int on);
+int gev_timing_reset(gev_ctx *ctx);
+/* Writes "name calls total_ms\n" lines into buf (NUL-terminated; truncated to
+   len). Synchronous. */
+int gev_timing_report(gev_ctx *ctx, char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,329 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (the HIP library in
+gevolution-1.2_amd/) never imports this module.
+
+Array conventions are those of gev_oracle.h: real fields are numpy float64
+arrays shaped (N, N, N) indexed [z, y, x]; tensor fields (6, N, N, N) in the
+order 00,01,02,11,12,22; vector fields (3, N, N, N); k-space fields complex128
+shaped (N, N, N//2+1) indexed [kz, ky, kx] (numpy rfftn layout); particles
+pos/q float64 (n, 3), id int64 (n,).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_hconf.restype = _d
+        L.orc_hconf.argtypes = [_d, _d, _p]
+        L.orc_rungekutta4bg.argtypes = [_p, _d, _p, _d]
+        L.orc_particle_horizon.restype = _d
+        L.orc_particle_horizon.argtypes = [_d, _d, _p]
+        L.orc_fft_r2c.argtypes = [_i, _p, _p]
+        L.orc_fft_c2r.argtypes = [_i, _p, _p]
+        L.orc_fft1d.argtypes = [_i, _p, _p, _i]
+        L.orc_solve_modified_poisson_ft.argtypes = [_i, _p, _p, _d, _d]
+        L.orc_project_ft_scalar.argtypes = [_i, _p, _p, _i]
+        L.orc_evolve_ft_vector.argtypes = [_i, _p, _p, _d]
+        L.orc_project_ft_vector.argtypes = [_i, _p, _p, _d, _d]
+        L.orc_prepare_ft_source_tensor.argtypes = [_i, _p, _p, _p, _d]
+        L.orc_prepare_ft_source_scalar.argtypes = [_i, _p, _p, _p, _d, _p, _d, _d, _d]
+        L.orc_file_particles.argtypes = [_i, _i64, _p, _p, _p, _p, _p, _p]
+        L.orc_projection_T00_project.argtypes = [_i, _i64, _p, _p, _p, _p, _d, _p, _d, _p, _d]
+        L.orc_projection_Tij_project.argtypes = [_i, _i64, _p, _p, _p, _p, _d, _p, _d, _p, _d]
+        L.orc_projection_T0i_project.argtypes = [_i, _i64, _p, _p, _p, _p, _d, _p, _p, _d]
+        L.orc_scalar_projection_cic_project.argtypes = [_i, _i64, _p, _p, _p, _d, _p, _d]
+        L.orc_update_vel.restype = _d
+        L.orc_update_vel.argtypes = [_i, _i64, _p, _p, _i, _d, _p, _p, _p, _i, _p]
+        L.orc_move_particles.argtypes = [_i, _i64, _p, _p, _i, _d, _p, _p, _p, _i, _p]
+        L.orc_next_dtau.restype = _d
+        L.orc_next_dtau.argtypes = [_p, _d]
+        L.orc_box_alloc.restype = _i
+        L.orc_box_alloc.argtypes = [_p, _i, _i64]
+        L.orc_box_free.argtypes = [_p]
+        L.orc_cycle.argtypes = [_p, _p, _p]
+        L.orc_set_threads.argtypes = [_i]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the oracle must be C-contiguous"
+    return a.ctypes.data
+
+
+class Cosmo(ctypes.Structure):
+    _fields_ = [("Omega_cdm", _d), ("Omega_b", _d), ("Omega_m", _d), ("Omega_Lambda", _d),
+                ("Omega_rad", _d), ("Omega_fld", _d), ("w0_fld", _d), ("wa_fld", _d), ("h", _d)]
+
+
+class Sim(ctypes.Structure):
+    _fields_ = [("N", _i), ("gr_flag", _i), ("vector_flag", _i), ("Cf", _d), ("steplimit", _d),
+                ("z_relax", _d), ("fourpiG", _d), ("cosmo", Cosmo)]
+
+
+class State(ctypes.Structure):
+    _fields_ = [("a", _d), ("tau", _d), ("dtau", _d), ("dtau_old", _d), ("maxvel", _d),
+                ("T00hom", _d), ("phi_k0", _d), ("cycle", _i)]
+
+
+class Box(ctypes.Structure):
+    _fields_ = [("n", _i64), ("pmass", _d), ("pos", _p), ("q", _p), ("id", _p),
+                ("phi", _p), ("chi", _p), ("source", _p), ("Sij", _p), ("Bi", _p),
+                ("scalarFT", _p), ("SijFT", _p), ("BiFT", _p),
+                ("cell", _p), ("count", _p), ("start", _p), ("perm", _p)]
+
+
+def make_cosmo(Omega_cdm=0.26, Omega_b=0.05, h=0.67, Omega_rad=0.0):
+    om = Omega_cdm + Omega_b
+    return Cosmo(Omega_cdm, Omega_b, om, 1.0 - om - Omega_rad, Omega_rad, 0.0, -1.0, 0.0, h)
+
+
+# ---------------------------------------------------------------- wrappers
+
+def set_threads(n):
+    lib().orc_set_threads(n)
+
+
+def hconf(a, fourpiG, cosmo):
+    return lib().orc_hconf(a, fourpiG, ctypes.byref(cosmo))
+
+
+def rk4bg(a, fourpiG, cosmo, dtau):
+    x = _d(a)
+    lib().orc_rungekutta4bg(ctypes.byref(x), fourpiG, ctypes.byref(cosmo), dtau)
+    return x.value
+
+
+def particle_horizon(a, fourpiG, cosmo):
+    return lib().orc_particle_horizon(a, fourpiG, ctypes.byref(cosmo))
+
+
+def fft_r2c(f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    N = f.shape[0]
+    out = np.zeros((N, N, N // 2 + 1), dtype=np.complex128)
+    lib().orc_fft_r2c(N, _ptr(f), _ptr(out))
+    return out
+
+
+def fft_c2r(k):
+    k = np.ascontiguousarray(k, dtype=np.complex128)
+    N = k.shape[0]
+    out = np.zeros((N, N, N), dtype=np.float64)
+    lib().orc_fft_c2r(N, _ptr(k), _ptr(out))
+    return out
+
+
+def fft1d(x, sign=-1):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    lib().orc_fft1d(x.shape[0], _ptr(x), _ptr(out), sign)
+    return out
+
+
+def solve_modified_poisson_ft(srcFT, coeff, modif=0.0):
+    srcFT = np.ascontiguousarray(srcFT, dtype=np.complex128)
+    out = np.zeros_like(srcFT)
+    lib().orc_solve_modified_poisson_ft(srcFT.shape[0], _ptr(srcFT), _ptr(out), coeff, modif)
+    return out
+
+
+def project_ft_scalar(SijFT, chiFT=None, add=0):
+    SijFT = np.ascontiguousarray(SijFT, dtype=np.complex128)
+    N = SijFT.shape[1]
+    out = np.zeros(SijFT.shape[1:], dtype=np.complex128) if chiFT is None else np.array(chiFT, dtype=np.complex128)
+    lib().orc_project_ft_scalar(N, _ptr(SijFT), _ptr(out), add)
+    return out
+
+
+def evolve_ft_vector(SijFT, BiFT, a2dtau):
+    SijFT = np.ascontiguousarray(SijFT, dtype=np.complex128)
+    out = np.array(BiFT, dtype=np.complex128, order="C")
+    lib().orc_evolve_ft_vector(SijFT.shape[1], _ptr(SijFT), _ptr(out), a2dtau)
+    return out
+
+
+def project_ft_vector(SiFT, coeff=1.0, modif=0.0):
+    SiFT = np.ascontiguousarray(SiFT, dtype=np.complex128)
+    out = np.zeros_like(SiFT)
+    lib().orc_project_ft_vector(SiFT.shape[1], _ptr(SiFT), _ptr(out), coeff, modif)
+    return out
+
+
+def prepare_ft_source_tensor(phi, Tij, coeff):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    Tij = np.ascontiguousarray(Tij, dtype=np.float64)
+    out = np.zeros_like(Tij)
+    lib().orc_prepare_ft_source_tensor(phi.shape[0], _ptr(phi), _ptr(Tij), _ptr(out), coeff)
+    return out
+
+
+def prepare_ft_source_scalar(phi, chi, source, bgmodel, coeff, coeff2, coeff3):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    source = np.ascontiguousarray(source, dtype=np.float64)
+    out = np.zeros_like(source)
+    lib().orc_prepare_ft_source_scalar(phi.shape[0], _ptr(phi), _ptr(chi), _ptr(source), bgmodel,
+                                       _ptr(out), coeff, coeff2, coeff3)
+    return out
+
+
+def file_particles(N, pos, ids):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    ids = np.ascontiguousarray(ids, dtype=np.int64)
+    n = pos.shape[0]
+    cell = np.zeros(max(n, 1), dtype=np.uint32)
+    count = np.zeros(N ** 3, dtype=np.uint32)
+    start = np.zeros(N ** 3 + 1, dtype=np.uint64)
+    perm = np.zeros(max(n, 1), dtype=np.int64)
+    lib().orc_file_particles(N, n, _ptr(pos), _ptr(ids), _ptr(cell), _ptr(count), _ptr(start), _ptr(perm))
+    return cell[:n], count, start, perm[:n]
+
+
+def _filing(N, pos, ids):
+    _, _, start, perm = file_particles(N, pos, ids)
+    if perm.size == 0:
+        perm = np.zeros(1, dtype=np.int64)
+    return start, perm
+
+
+def projection_T00_project(N, pos, q, ids, pmass, a=1.0, phi=None, coeff=1.0, out=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    start, perm = _filing(N, pos, ids)
+    out = np.zeros((N, N, N)) if out is None else out
+    phi = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+    lib().orc_projection_T00_project(N, pos.shape[0], _ptr(pos), _ptr(q), _ptr(start), _ptr(perm),
+                                     pmass, _ptr(out), a, _ptr(phi), coeff)
+    return out
+
+
+def projection_Tij_project(N, pos, q, ids, pmass, a=1.0, phi=None, coeff=1.0, out=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    start, perm = _filing(N, pos, ids)
+    out = np.zeros((6, N, N, N)) if out is None else out
+    phi = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+    lib().orc_projection_Tij_project(N, pos.shape[0], _ptr(pos), _ptr(q), _ptr(start), _ptr(perm),
+                                     pmass, _ptr(out), a, _ptr(phi), coeff)
+    return out
+
+
+def projection_T0i_project(N, pos, q, ids, pmass, phi=None, coeff=1.0, out=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    start, perm = _filing(N, pos, ids)
+    out = np.zeros((3, N, N, N)) if out is None else out
+    phi = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+    lib().orc_projection_T0i_project(N, pos.shape[0], _ptr(pos), _ptr(q), _ptr(start), _ptr(perm),
+                                     pmass, _ptr(out), _ptr(phi), coeff)
+    return out
+
+
+def scalar_projection_cic_project(N, pos, ids, pmass, coeff=1.0, out=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    start, perm = _filing(N, pos, ids)
+    out = np.zeros((N, N, N)) if out is None else out
+    lib().orc_scalar_projection_cic_project(N, pos.shape[0], _ptr(pos), _ptr(start), _ptr(perm),
+                                            pmass, _ptr(out), coeff)
+    return out
+
+
+Q_GR, Q_NEWTON = 0, 1
+POS_GR, POS_NEWTON = 0, 1
+
+
+def update_vel(N, pos, q, method, dtau, phi, chi, Bi, nfield, params):
+    """Returns (new q, sqrt(max v^2/a^2))."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    qn = np.array(q, dtype=np.float64, order="C")
+    prm = np.asarray(params, dtype=np.float64)
+    f = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (phi, chi, Bi)]
+    m = lib().orc_update_vel(N, pos.shape[0], _ptr(pos), _ptr(qn), method, dtau,
+                             _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), nfield, _ptr(prm))
+    return qn, m
+
+
+def move_particles(N, pos, q, method, dtau, phi, chi, Bi, nfield, params):
+    posn = np.array(pos, dtype=np.float64, order="C")
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    prm = np.asarray(params, dtype=np.float64)
+    f = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (phi, chi, Bi)]
+    lib().orc_move_particles(N, posn.shape[0], _ptr(posn), _ptr(q), method, dtau,
+                             _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), nfield, _ptr(prm))
+    return posn
+
+
+class OracleBox:
+    """Owns an orc_box and exposes its arrays as numpy views."""
+
+    def __init__(self, N, n):
+        self.N, self.n = N, n
+        self.box = Box()
+        if lib().orc_box_alloc(ctypes.byref(self.box), N, n) != 0:
+            raise MemoryError("orc_box_alloc failed")
+        H = N // 2 + 1
+
+        def view(addr, shape, dtype):
+            count = int(np.prod(shape))
+            buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(addr)
+            return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+        b = self.box
+        self.pos = view(b.pos, (n, 3), np.float64)
+        self.q = view(b.q, (n, 3), np.float64)
+        self.id = view(b.id, (n,), np.int64)
+        self.phi = view(b.phi, (N, N, N), np.float64)
+        self.chi = view(b.chi, (N, N, N), np.float64)
+        self.source = view(b.source, (N, N, N), np.float64)
+        self.Sij = view(b.Sij, (6, N, N, N), np.float64)
+        self.Bi = view(b.Bi, (3, N, N, N), np.float64)
+        self.scalarFT = view(b.scalarFT, (N, N, H), np.complex128)
+        self.SijFT = view(b.SijFT, (6, N, N, H), np.complex128)
+        self.BiFT = view(b.BiFT, (3, N, N, H), np.complex128)
+        self.cell = view(b.cell, (n,), np.uint32)
+        self.count = view(b.count, (N ** 3,), np.uint32)
+        self.start = view(b.start, (N ** 3 + 1,), np.uint64)
+        self.perm = view(b.perm, (n,), np.int64)
+
+    def file(self):
+        lib().orc_file_particles(self.N, self.n, self.box.pos, self.box.id, self.box.cell,
+                                 self.box.count, self.box.start, self.box.perm)
+
+    def cycle(self, sim, state):
+        lib().orc_cycle(ctypes.byref(sim), ctypes.byref(self.box), ctypes.byref(state))
+
+    def close(self):
+        if self.box.pos:
+            lib().orc_box_free(ctypes.byref(self.box))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
