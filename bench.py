@@ -1,0 +1,278 @@
+"""Throughput of the GR particle-mesh cycle on one MI355X.
+
+Workload (the plain FP64 GPU drop-in configuration): GR gravity, 512^3 grid,
+512^3 CDM particles, box 2048 Mpc/h, h = 0.67, Omega_cdm = 0.26, Omega_b =
+0.05 folded into CDM, Courant factor 48, time step limit 0.04, vector method
+parabolic, FP64, z_in = 100.  A "step" is one cycle of gevolution's main
+loop (main.cpp:372-879): T00+Tij deposit, both prepareFTsource stencils, the
+12 FFTs with the k-space solves, and the fused kick + drift + re-filing.
+
+Initial state: particles on the cell-centred lattice (the reference's sc1
+crystal template tiled N/4 times) with a uniform random jitter of +-0.05
+cells and small random momenta; phi = chi = B = 0.  This is synthetic data
+of the configuration's shape; per-cycle cost does not depend on the field
+values at these redshifts.
+
+Output: one JSON line.  `value` is particle-steps/s with everything resident
+in HBM; `roofline` is the kernel with the largest share of the timed region,
+its algorithmic bytes per launch divided by its mean duration (HIP events on
+the context stream); `cpu_baseline` is the CPU oracle (the restated
+reference algorithm, OpenMP) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gevolution-1.2_amd"))
+
+METRIC = "GR PM particle-steps/s at 512^3 (%HBM peak); p99 ensemble latency vs plain"
+HBM_PEAK_GBS = 8000.0
+C_LIGHT = 2997.92458
+
+
+def algorithmic_bytes(name, N, Np):
+    """Bytes each kernel must move at least once per launch (FP64 form):
+    every input read once, every output written once."""
+    n3 = N ** 3
+    nk = N * N * (N // 2 + 1)
+    table = {
+        "kick_drift": 96 * Np + 5 * 8 * n3,                 # pos,q r+w; phi, chi, B(3) read
+        "update_vel": 72 * Np + 5 * 8 * n3,
+        "move_particles": 72 * Np + 5 * 8 * n3,
+        "projection_T00_Tij_project": 48 * Np + 8 * 8 * n3,  # pos,q read; phi read; T00 + Tij(6) written
+        "projection_T00_project": 48 * Np + 2 * 8 * n3,
+        "projection_Tij_project": 48 * Np + 7 * 8 * n3,
+        "prepare_ft_source_tensor": 13 * 8 * n3,            # phi + Tij(6) read, Sij(6) written
+        "prepare_ft_source_scalar": 4 * 8 * n3,             # phi, chi, source read, source written
+        "fft_r2c_x": 8 * n3 + 16 * nk,
+        "fft_c2r_x": 16 * nk + 8 * n3,
+        "fft_c2c_y": 32 * nk,
+        "fft_c2c_z": 32 * nk,
+        "fft_c2c_y_inv": 32 * nk,
+        "fft_c2c_z_inv": 32 * nk,
+        "solve_modified_poisson_ft": 32 * nk,
+        "project_scalar_evolve_vector": 13 * 16 * nk,       # SijFT(6)+BiFT(3) read, chiFT+BiFT(3) written
+        "field_sum": 8 * n3,
+        "filing_scan": 8 * n3,
+        "filing_scatter": 12 * Np,
+        "filing_sort": 4 * n3,
+    }
+    return table.get(name)
+
+
+# algorithmic bytes of one whole GR-parabolic cycle at the fusion boundaries of
+# this implementation (FP64 fields, FP64 particles)
+def cycle_bytes(N, Np):
+    n3 = N ** 3
+    nk = N * N * (N // 2 + 1)
+    fft_f = 8 * n3 + 16 * nk
+    return (48 * Np + 64 * n3          # deposit T00 + Tij
+            + 32 * n3                  # prepare scalar (+ T00hom reads source: 8 more)
+            + 8 * n3                   # T00hom
+            + fft_f + 32 * nk + fft_f  # source r2c, Poisson, phi c2r
+            + 104 * n3                 # prepare tensor
+            + 6 * fft_f                # Sij r2c
+            + 13 * 16 * nk             # projectFTscalar + evolveFTvector
+            + fft_f + 3 * fft_f        # chi c2r, Bi c2r
+            + 96 * Np + 40 * n3        # kick + drift
+            + 8 * n3)                  # cell key + counts
+
+
+def measured_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 --pmc summary
+    (lines 'kernel read_bytes written_bytes traffic_bytes'), or None."""
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == kernel and parts[3].isdigit():
+                return int(parts[3])
+    except OSError:
+        pass
+    return None
+
+
+def synthetic_state(N, seed):
+    rng = np.random.default_rng(seed)
+    Np = N ** 3
+    g = (np.arange(N, dtype=np.float64) + 0.5) / N
+    pos = np.empty((Np, 3))
+    pos[:, 0] = np.tile(g, N * N)
+    pos[:, 1] = np.tile(np.repeat(g, N), N)
+    pos[:, 2] = np.repeat(g, N * N)
+    pos += rng.uniform(-0.05 / N, 0.05 / N, size=pos.shape)
+    q = rng.standard_normal(size=(Np, 3)) * 2e-6
+    ids = np.arange(Np, dtype=np.int64)
+    return pos, q, ids
+
+
+def sim_setup(gev, N, boxsize):
+    cosmo = gev.make_cosmology(0.26, 0.05, 0.67)
+    fourpiG = 1.5 * boxsize * boxsize / C_LIGHT / C_LIGHT
+    sim = gev.SimParams(1, 0, 48.0, 0.04, 100.0, fourpiG, cosmo)
+    a0 = 1.0 / 101.0
+    st = gev.CycleState(a0, 0.0, gev.next_dtau(sim, N, a0), 0.0, 0.0, 0.0, 0.0, 0, 0)
+    return sim, st
+
+
+def cpu_baseline(N_sample, cycles, threads):
+    """Times the CPU oracle on a bounded sample: an N_sample^3 box with the
+    same cell size and settings, `cycles` timed cycles after one warm-up."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.build()
+    orc.set_threads(threads)
+    N = N_sample
+    pos, q, ids = synthetic_state(N, 7)
+    ob = orc.OracleBox(N, N ** 3)
+    ob.pos[:] = pos
+    ob.q[:] = q
+    ob.id[:] = ids
+    ob.box.pmass = 0.31 / N ** 3
+    del pos, q, ids
+    ob.file()
+    boxsize = 4.0 * N
+    sim = orc.Sim(N, 1, 0, 48.0, 0.04, 100.0, 1.5 * boxsize ** 2 / C_LIGHT ** 2, orc.make_cosmo())
+    a0 = 1.0 / 101.0
+    st = orc.State(a0, 0.0, orc.lib().orc_next_dtau(orc.ctypes.byref(sim), a0), 0.0, 0.0, 0.0, 0.0, 0)
+    ob.cycle(sim, st)  # warm-up (cycle 0 skips the phi update)
+    t0 = time.perf_counter()
+    for _ in range(cycles):
+        ob.cycle(sim, st)
+    dt = time.perf_counter() - t0
+    ob.close()
+    return N ** 3 * cycles / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ngrid", type=int, default=512)
+    ap.add_argument("--boxsize", type=float, default=None, help="Mpc/h (default 4 * ngrid)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-ngrid", type=int, default=256)
+    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-timers", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_gr_cycle_512_fp64.txt"),
+                    help="measured HBM bytes per launch (rocprofv3 --pmc summary)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 5 plain 1 GiB device copies (scale for the PMC byte counters)")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+        return
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs one box on one GPU; multi-GPU runs are not part of this build "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import gevpm as gev
+    gev.lib()
+
+    N = args.ngrid
+    Np = N ** 3
+    boxsize = args.boxsize if args.boxsize else 4.0 * N
+    t_setup = time.perf_counter()
+    pos, q, ids = synthetic_state(N, 7)
+    box = gev.Box(N, Np)
+    box.upload_particles(pos, q, ids, 0.31 / Np)
+    del pos, q, ids
+    sim, st = sim_setup(gev, N, boxsize)
+    box.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    print("setup %.1f s (N=%d, box %.0f Mpc/h)" % (t_setup, N, boxsize), file=sys.stderr, flush=True)
+
+    box.run_cycles(sim, st, args.warmup)
+    box.synchronize()
+    if not args.no_timers:
+        box.timing_reset()
+        box.timing(True)
+    t0 = time.perf_counter()
+    box.run_cycles(sim, st, args.steps)
+    box.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernels = box.timing_report() if not args.no_timers else {}
+    box.timing(False)
+    print("timed %d cycles in %.3f s; a = %.5f, T00hom = %.6f, maxvel = %.3e"
+          % (args.steps, elapsed, st.a, st.T00hom, st.maxvel), file=sys.stderr, flush=True)
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = Np * args.steps / elapsed
+    step_bytes = cycle_bytes(N, Np)
+
+    roofline = None
+    breakdown = {}
+    if kernels:
+        total_ms = sum(ms for _, ms in kernels.values())
+        for name, (calls, ms) in sorted(kernels.items(), key=lambda kv: -kv[1][1]):
+            avg = ms / calls
+            b = algorithmic_bytes(name, N, Np)
+            breakdown[name] = {"calls_per_step": calls / args.steps, "ms_per_launch": round(avg, 4),
+                               "share": round(ms / total_ms, 4),
+                               "GBps": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+        dom, (calls, ms) = max(kernels.items(), key=lambda kv: kv[1][1])
+        avg_s = ms / calls * 1e-3
+        b = algorithmic_bytes(dom, N, Np)
+        traffic = measured_traffic(args.traffic, dom) if N == 512 else None
+        ach = b / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algorithmic_bytes": b, "ms_per_launch": round(avg_s * 1e3, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        v, dt = cpu_baseline(args.cpu_ngrid, args.cpu_cycles, threads)
+        cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": "CPU oracle (FP64 C restatement of the reference cycle, OpenMP) on a %d^3 grid "
+                         "with %d^3 particles, box %.0f Mpc/h (same cell size and settings), %d timed "
+                         "GR cycles after 1 warm-up, %.1f s" % (args.cpu_ngrid, args.cpu_ngrid,
+                                                                4.0 * args.cpu_ngrid, args.cpu_cycles, dt)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "GR PM cycle (main.cpp:372-879), %d^3 grid, %d^3 CDM particles, box %.0f Mpc/h, "
+                               "LCDM h=0.67 Om=0.31, Cf 48, steplimit 0.04, parabolic B, FP64 plain form"
+                               % (N, N, boxsize),
+                   "ngrid": N, "particles": Np, "boxsize_mpc_h": boxsize, "precision": "fp64"},
+        "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
+                          "achieved": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": breakdown,
+    }
+    box.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
