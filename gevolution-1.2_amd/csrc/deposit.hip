@@ -1,0 +1,550 @@
+// deposit.hip -- CIC projections of one box (gevolution.hpp:926-1297 and
+// LATfield2's scalarProjectionCIC_project) as an owner-computes gather.
+//
+// A workgroup owns a tile of 16x4x4 target cells; its particles come from the
+// 17x5x5 source cells [x0-1, x0+16) x [y0-1, y0+4) x [z0-1, z0+4) (periodic).
+// The source cells are enumerated in ascending global site index and their
+// particles (each cell's run of the filing permutation, ascending ID) are
+// staged into LDS as per-particle records -- CIC weights and the energy /
+// stress factors, computed once per particle instead of once per target.
+// Then every thread sums, for its target cell, the contributions of its 8
+// source cells in ascending site index, exactly the order in which the
+// reference's sequential site loop adds its local cubes onto that target
+// (gevolution.hpp:1012-1019, 1129-1144, 1277-1294).  The target is written
+// once; there are no atomics, and the result is bit-identical to the
+// sequential scatter for the same in-cell particle order.
+//
+// Tiles whose source cells hold more particles than the LDS holds are
+// processed in chunks of whole source cells (still in site order); a single
+// cell larger than a chunk is read directly from global memory.
+#include "gev_internal.h"
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int TX = 16, TY = 4, TZ = 4;
+constexpr int SX = TX + 1, SY = TY + 1, SZ = TZ + 1;
+constexpr int NSRC = SX * SY * SZ;  // 425 source cells
+constexpr int CAP = 448;            // particle records per chunk (3 workgroups per CU)
+
+// cell index; N^3 <= 2^30 fits 32 bits
+__device__ __forceinline__ uint32_t L(int N, int x, int y, int z) {
+    return (uint32_t)x + (uint32_t)N * ((uint32_t)y + (uint32_t)N * (uint32_t)z);
+}
+
+// x / dx with dx = 1/N.  For N a power of two dx is exact and the quotient
+// equals x * N bit for bit, so the multiply replaces the division.
+template <bool POW2> __device__ __forceinline__ double over_dx(double x, double dx, double Nd) {
+    return POW2 ? x * Nd : x / dx;
+}
+__device__ __forceinline__ int wrapi(int i, int N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
+
+// Source slot s along one axis is coordinate lo0 - 1 + s (wrapped).  In
+// ascending global order the slots come as 0,1,..,n-1 unless lo0 == 0, where
+// slot 0 (coordinate N-1) comes last.
+__device__ __forceinline__ int order_pos(int s, int lo0, int n) { return lo0 > 0 ? s : (s == 0 ? n - 1 : s - 1); }
+__device__ __forceinline__ int order_slot(int p, int lo0, int n) { return lo0 > 0 ? p : (p == n - 1 ? 0 : p + 1); }
+
+template <int KIND> struct Kind {
+    static constexpr bool T00 = KIND == 1 || KIND == 3 || KIND == 5;
+    static constexpr bool TIJ = KIND == 2 || KIND == 3;
+    static constexpr bool T0I = KIND == 4;
+    static constexpr bool QE = (T00 && KIND != 5) || TIJ;  // needs e = sqrt(q^2 + a^2)
+};
+
+// per-particle record: everything a target needs from one particle
+struct Rec {
+    double up0, up1, up2;      // (pos - source corner) / dx
+    double e, f00;             // T00: e and f (gevolution.hpp:989-991)
+    double fij;                // Tij: f (gevolution.hpp:1238)
+    double w0, w1, w2;         // Tij: m q_i q_i / e   |  T0i: m q_i
+    double w01, w02, w12;      // Tij: m q_i q_j / e
+};
+
+// position and momentum of one particle
+struct PQ {
+    double x, y, z, q0, q1, q2;
+};
+
+template <int KIND>
+__device__ __forceinline__ PQ load_pq(const double *__restrict__ pos, const double *__restrict__ q, int64_t cap,
+                                      uint32_t p) {
+    using K = Kind<KIND>;
+    PQ v;
+    v.x = pos[p];
+    v.y = pos[cap + p];
+    v.z = pos[2 * cap + p];
+    v.q0 = v.q1 = v.q2 = 0.;
+    if (K::QE || K::T0I) {
+        v.q0 = q[p];
+        v.q1 = q[cap + p];
+        v.q2 = q[2 * cap + p];
+    }
+    return v;
+}
+
+template <int KIND, bool USE_PHI, bool POW2>
+__device__ __forceinline__ Rec make_rec(const PQ &v, double rx, double ry, double rz, double dx, double Nd,
+                                        double a, double mass_t) {
+    using K = Kind<KIND>;
+    Rec r;
+    r.up0 = over_dx<POW2>(v.x - rx, dx, Nd);
+    r.up1 = over_dx<POW2>(v.y - ry, dx, Nd);
+    r.up2 = over_dx<POW2>(v.z - rz, dx, Nd);
+    r.e = a;
+    r.f00 = 0.;
+    r.fij = 0.;
+    r.w0 = r.w1 = r.w2 = r.w01 = r.w02 = r.w12 = 0.;
+    const double q0 = v.q0, q1 = v.q1, q2 = v.q2;
+    if (K::QE) {
+        const double fq = q0 * q0 + q1 * q1 + q2 * q2;
+        const double e = sqrt(fq + a * a);
+        if (K::T00 && KIND != 5 && USE_PHI) {
+            r.e = e;
+            r.f00 = 3. * e + fq / e;
+        }
+        if (K::TIJ) {
+            r.fij = 4. + a * a / (fq + a * a);
+            r.w0 = mass_t * q0 * q0 / e;
+            r.w1 = mass_t * q1 * q1 / e;
+            r.w2 = mass_t * q2 * q2 / e;
+            r.w01 = mass_t * q0 * q1 / e;
+            r.w02 = mass_t * q0 * q2 / e;
+            r.w12 = mass_t * q1 * q2 / e;
+        }
+    }
+    if (K::T0I) {
+        r.w0 = mass_t * q0;
+        r.w1 = mass_t * q1;
+        r.w2 = mass_t * q2;
+    }
+    return r;
+}
+
+// LDS image of the records (structure of arrays)
+template <int KIND> struct Stage {
+    double up0[CAP], up1[CAP], up2[CAP];
+    double e[Kind<KIND>::T00 ? CAP : 1], f00[Kind<KIND>::T00 ? CAP : 1];
+    double fij[Kind<KIND>::TIJ ? CAP : 1];
+    double w0[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
+    double w1[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
+    double w2[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
+    double w01[Kind<KIND>::TIJ ? CAP : 1], w02[Kind<KIND>::TIJ ? CAP : 1], w12[Kind<KIND>::TIJ ? CAP : 1];
+
+    __device__ __forceinline__ void put(int i, const Rec &r) {
+        using K = Kind<KIND>;
+        up0[i] = r.up0;
+        up1[i] = r.up1;
+        up2[i] = r.up2;
+        if (K::T00) {
+            e[i] = r.e;
+            f00[i] = r.f00;
+        }
+        if (K::TIJ) fij[i] = r.fij;
+        if (K::TIJ || K::T0I) {
+            w0[i] = r.w0;
+            w1[i] = r.w1;
+            w2[i] = r.w2;
+        }
+        if (K::TIJ) {
+            w01[i] = r.w01;
+            w02[i] = r.w02;
+            w12[i] = r.w12;
+        }
+    }
+    __device__ __forceinline__ Rec get(int i) const {
+        using K = Kind<KIND>;
+        Rec r;
+        r.up0 = up0[i];
+        r.up1 = up1[i];
+        r.up2 = up2[i];
+        r.e = K::T00 ? e[i] : 0.;
+        r.f00 = K::T00 ? f00[i] : 0.;
+        r.fij = K::TIJ ? fij[i] : 0.;
+        r.w0 = (K::TIJ || K::T0I) ? w0[i] : 0.;
+        r.w1 = (K::TIJ || K::T0I) ? w1[i] : 0.;
+        r.w2 = (K::TIJ || K::T0I) ? w2[i] : 0.;
+        r.w01 = K::TIJ ? w01[i] : 0.;
+        r.w02 = K::TIJ ? w02[i] : 0.;
+        r.w12 = K::TIJ ? w12[i] : 0.;
+        return r;
+    }
+};
+
+// target-side values and the running per-source partial sums
+struct Target {
+    double ph, plq01, plq02, plq12, lnk0, lnk1, lnk2;
+};
+
+struct Partial {
+    double lc, ld0, ld1, ld2, lo01, lo02, lo12, lv0, lv1, lv2;
+    __device__ __forceinline__ void clear() { lc = ld0 = ld1 = ld2 = lo01 = lo02 = lo12 = lv0 = lv1 = lv2 = 0.; }
+};
+
+// one particle's contribution to one target (expression order of
+// gevolution.hpp:995-1009, 1241-1272, 1107-1126)
+template <int KIND>
+__device__ __forceinline__ void add_particle(Partial &P, const Rec &r, const Target &T, int dxx, int dy,
+                                             int dz) {
+    using K = Kind<KIND>;
+    const double W0 = dxx ? r.up0 : 1.0 - r.up0;
+    const double W1 = dy ? r.up1 : 1.0 - r.up1;
+    const double W2 = dz ? r.up2 : 1.0 - r.up2;
+    if (K::T00) P.lc += W0 * W1 * W2 * (r.e + r.f00 * T.ph);
+    if (K::TIJ) {
+        P.ld0 += r.w0 * W0 * W1 * W2 * (1. + r.fij * T.ph);
+        P.ld1 += r.w1 * W0 * W1 * W2 * (1. + r.fij * T.ph);
+        P.ld2 += r.w2 * W0 * W1 * W2 * (1. + r.fij * T.ph);
+        if (dxx == 0 && dy == 0) P.lo01 += r.w01 * W2 * (1. + r.fij * 0.25 * T.plq01);
+        if (dxx == 0 && dz == 0) P.lo02 += r.w02 * W1 * (1. + r.fij * 0.25 * T.plq02);
+        if (dy == 0 && dz == 0) P.lo12 += r.w12 * W0 * (1. + r.fij * 0.25 * T.plq12);
+    }
+    if (K::T0I) {
+        if (dxx == 0) P.lv0 += r.w0 * W1 * W2;
+        if (dy == 0) P.lv1 += r.w1 * W0 * W2;
+        if (dz == 0) P.lv2 += r.w2 * W0 * W1;
+    }
+}
+
+struct Acc {
+    double s00, d0, d1, d2, o01, o02, o12, v0, v1, v2;
+};
+
+template <int KIND>
+__device__ __forceinline__ void add_source(Acc &A, const Partial &P, const Target &T, int dxx, int dy,
+                                           int dz, double mass_s) {
+    using K = Kind<KIND>;
+    if (K::T00) A.s00 += P.lc * mass_s;
+    if (K::TIJ) {
+        A.d0 += P.ld0;
+        A.d1 += P.ld1;
+        A.d2 += P.ld2;
+        if (dxx == 0 && dy == 0) A.o01 += P.lo01;
+        if (dxx == 0 && dz == 0) A.o02 += P.lo02;
+        if (dy == 0 && dz == 0) A.o12 += P.lo12;
+    }
+    if (K::T0I) {
+        if (dxx == 0) A.v0 += P.lv0 * T.lnk0;
+        if (dy == 0) A.v1 += P.lv1 * T.lnk1;
+        if (dz == 0) A.v2 += P.lv2 * T.lnk2;
+    }
+}
+
+// does source offset d feed any component of this KIND at all?
+template <int KIND> __device__ __forceinline__ bool feeds(int dxx, int dy, int dz) {
+    if (KIND == 4) return dxx == 0 || dy == 0 || dz == 0;
+    return true;
+}
+
+template <int KIND, bool USE_PHI, bool ACCUM, bool POW2>
+__global__ __launch_bounds__(TPB) void k_deposit_tiled(
+    const double *__restrict__ pos, const double *__restrict__ q, int64_t cap,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
+    const double *__restrict__ phi, double *__restrict__ out_s, double *__restrict__ out_t, int N,
+    int ntx, int nty, int ntz, double a, double mass_s, double mass_t) {
+    using K = Kind<KIND>;
+    __shared__ Stage<KIND> st;
+    __shared__ uint32_t s_off[NSRC + 1];
+    __shared__ uint32_t s_gbeg[NSRC];
+    __shared__ uint32_t s_cnt[NSRC];
+    __shared__ uint32_t s_scan[TPB];
+    __shared__ uint16_t s_rc[CAP];
+
+    // XCD-aware tile order: consecutive workgroups of one XCD (dispatched
+    // round-robin over the 8 XCDs) take neighbouring tiles and share L2
+    const uint32_t nblk = (uint32_t)ntx * nty * ntz;
+    uint32_t tile = blockIdx.x;
+    if (nblk % 8 == 0) tile = (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;
+    const int tx = (int)(tile % ntx), ty = (int)((tile / ntx) % nty), tz = (int)(tile / ((uint32_t)ntx * nty));
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+    const int nx = min(TX, N - x0), ny = min(TY, N - y0), nz = min(TZ, N - z0);
+    const int nsx = nx + 1, nsy = ny + 1, nsz = nz + 1;
+    const int nsrc = nsx * nsy * nsz;
+    const double dx = 1.0 / (double)N, Nd = (double)N;
+    const int64_t n3v = (int64_t)N * N * N;
+
+    // 1. source cells in ascending site order: CSR begin and count
+    for (int c = threadIdx.x; c < nsrc; c += TPB) {
+        const int px = c % nsx, py = (c / nsx) % nsy, pz = c / (nsx * nsy);
+        const int gx = wrapi(x0 - 1 + order_slot(px, x0, nsx), N);
+        const int gy = wrapi(y0 - 1 + order_slot(py, y0, nsy), N);
+        const int gz = wrapi(z0 - 1 + order_slot(pz, z0, nsz), N);
+        const uint32_t g = L(N, gx, gy, gz);
+        const uint32_t b = start[g];
+        s_gbeg[c] = b;
+        s_cnt[c] = start[g + 1] - b;
+    }
+    __syncthreads();
+    // exclusive scan of the counts (two cells per thread)
+    {
+        const int c = 2 * threadIdx.x;
+        const uint32_t c0 = c < nsrc ? s_cnt[c] : 0u, c1 = c + 1 < nsrc ? s_cnt[c + 1] : 0u;
+        const uint32_t v = c0 + c1;
+        s_scan[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < TPB; off <<= 1) {
+            const uint32_t u = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s_scan[threadIdx.x] += u;
+            __syncthreads();
+        }
+        const uint32_t excl = s_scan[threadIdx.x] - v;
+        if (c < nsrc) s_off[c] = excl;
+        if (c + 1 < nsrc) s_off[c + 1] = excl + c0;
+        if (threadIdx.x == TPB - 1) s_off[nsrc] = s_scan[TPB - 1];
+        __syncthreads();
+    }
+
+    // 2. this thread's target (one per thread; the tile has TPB cells)
+    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+    const bool active = lx < nx && ly < ny && lz < nz;
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    Target T;
+    T.ph = T.plq01 = T.plq02 = T.plq12 = 0.;
+    T.lnk0 = T.lnk1 = T.lnk2 = 1.;
+    Acc A;
+    A.s00 = A.d0 = A.d1 = A.d2 = A.o01 = A.o02 = A.o12 = A.v0 = A.v1 = A.v2 = 0.;
+    const uint32_t t = active ? L(N, x, y, z) : 0u;
+    if (active) {
+        double phx = 0., phy = 0., phz = 0., phxy = 0., phxz = 0., phyz = 0.;
+        if (USE_PHI) {
+            const int xp = x == N - 1 ? 0 : x + 1, yp = y == N - 1 ? 0 : y + 1, zp = z == N - 1 ? 0 : z + 1;
+            T.ph = phi[t];
+            phx = phi[L(N, xp, y, z)];
+            phy = phi[L(N, x, yp, z)];
+            phz = phi[L(N, x, y, zp)];
+            phxy = phi[L(N, xp, yp, z)];
+            phxz = phi[L(N, xp, y, zp)];
+            phyz = phi[L(N, x, yp, zp)];
+        }
+        // plaquette sums (gevolution.hpp:1263-1272) and link factors (1129-1144)
+        T.plq01 = T.ph + phy + phx + phxy;
+        T.plq02 = T.ph + phz + phx + phxz;
+        T.plq12 = T.ph + phz + phy + phyz;
+        T.lnk0 = 1. + T.ph + phx;
+        T.lnk1 = 1. + T.ph + phy;
+        T.lnk2 = 1. + T.ph + phz;
+        if (ACCUM) {
+            if (K::T00) A.s00 = out_s[t];
+            if (K::TIJ) {
+                A.d0 = out_t[t];
+                A.d1 = out_t[3 * n3v + t];
+                A.d2 = out_t[5 * n3v + t];
+                A.o01 = out_t[n3v + t];
+                A.o02 = out_t[2 * n3v + t];
+                A.o12 = out_t[4 * n3v + t];
+            }
+            if (K::T0I) {
+                A.v0 = out_t[t];
+                A.v1 = out_t[n3v + t];
+                A.v2 = out_t[2 * n3v + t];
+            }
+        }
+    }
+    // enumeration index of each of the 8 sources, visited in site order
+    int csrc[8], dsrc[8];
+    {
+        int k = 0;
+        for (int iz = 0; iz < 2; iz++) {
+            const int dz = z > 0 ? 1 - iz : iz;
+            for (int iy = 0; iy < 2; iy++) {
+                const int dy = y > 0 ? 1 - iy : iy;
+                for (int ix = 0; ix < 2; ix++) {
+                    const int dxx = x > 0 ? 1 - ix : ix;
+                    const int px = order_pos(lx + 1 - dxx, x0, nsx);
+                    const int py = order_pos(ly + 1 - dy, y0, nsy);
+                    const int pz = order_pos(lz + 1 - dz, z0, nsz);
+                    csrc[k] = (pz * nsy + py) * nsx + px;
+                    dsrc[k] = dxx | (dy << 1) | (dz << 2);
+                    k++;
+                }
+            }
+        }
+    }
+
+    // 3. chunks of whole source cells, in site order (one chunk in the
+    //    common case: the 425 source cells hold fewer than CAP particles)
+    int cb = 0;
+    while (cb < nsrc) {
+        const uint32_t base = s_off[cb];
+        int ce;
+        bool direct = false;
+        if (s_off[cb + 1] - base > (uint32_t)CAP) {
+            ce = cb + 1;
+            direct = true;
+        } else if (s_off[nsrc] - base <= (uint32_t)CAP) {
+            ce = nsrc;
+        } else {
+            int lo = cb + 1, hi = nsrc;  // largest ce with s_off[ce] - base <= CAP
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) / 2;
+                if (s_off[mid] - base <= (uint32_t)CAP) lo = mid;
+                else hi = mid - 1;
+            }
+            ce = lo;
+        }
+        if (!direct) {
+            // record -> source cell map, then one thread per record with all
+            // permutation loads, then all position/momentum loads, in flight
+            const uint32_t nrec = s_off[ce] - base;
+            for (int c = cb + threadIdx.x; c < ce; c += TPB) {
+                const uint32_t r0 = s_off[c] - base, n = s_cnt[c];
+                for (uint32_t i = 0; i < n; i++) s_rc[r0 + i] = (uint16_t)c;
+            }
+            __syncthreads();
+            constexpr int PR = (CAP + TPB - 1) / TPB;
+            uint32_t pidx[PR];
+            int pc[PR];
+#pragma unroll
+            for (int k = 0; k < PR; k++) {
+                const uint32_t r = threadIdx.x + k * TPB;
+                pc[k] = 0;
+                pidx[k] = 0;
+                if (r < nrec) {
+                    pc[k] = s_rc[r];
+                    pidx[k] = perm[s_gbeg[pc[k]] + (r + base - s_off[pc[k]])];
+                }
+            }
+            PQ pq[PR];
+#pragma unroll
+            for (int k = 0; k < PR; k++)
+                if (threadIdx.x + k * TPB < nrec) pq[k] = load_pq<KIND>(pos, q, cap, pidx[k]);
+#pragma unroll
+            for (int k = 0; k < PR; k++) {
+                const uint32_t r = threadIdx.x + k * TPB;
+                if (r < nrec) {
+                    const int c = pc[k];
+                    const int px = c % nsx, py = (c / nsx) % nsy, pz = c / (nsx * nsy);
+                    const double rx = wrapi(x0 - 1 + order_slot(px, x0, nsx), N) * dx;
+                    const double ry = wrapi(y0 - 1 + order_slot(py, y0, nsy), N) * dx;
+                    const double rz = wrapi(z0 - 1 + order_slot(pz, z0, nsz), N) * dx;
+                    st.put((int)r, make_rec<KIND, USE_PHI, POW2>(pq[k], rx, ry, rz, dx, Nd, a, mass_t));
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll 1
+            for (int k = 0; k < 8; k++) {
+                const int c = csrc[k];
+                if (c < cb || c >= ce) continue;
+                const int dxx = dsrc[k] & 1, dy = (dsrc[k] >> 1) & 1, dz = dsrc[k] >> 2;
+                if (!feeds<KIND>(dxx, dy, dz)) continue;
+                const uint32_t n = s_cnt[c];
+                if (n == 0) continue;
+                Partial P;
+                P.clear();
+                if (!direct) {
+                    const uint32_t r0 = s_off[c] - base;
+                    for (uint32_t i = 0; i < n; i++) add_particle<KIND>(P, st.get((int)(r0 + i)), T, dxx, dy, dz);
+                } else {
+                    const double rx = wrapi(x - dxx, N) * dx, ry = wrapi(y - dy, N) * dx, rz = wrapi(z - dz, N) * dx;
+                    for (uint32_t i = 0; i < n; i++)
+                        add_particle<KIND>(P, make_rec<KIND, USE_PHI, POW2>(load_pq<KIND>(pos, q, cap, perm[s_gbeg[c] + i]),
+                                                                            rx, ry, rz, dx, Nd, a, mass_t),
+                                           T, dxx, dy, dz);
+                }
+                add_source<KIND>(A, P, T, dxx, dy, dz, mass_s);
+            }
+        }
+        __syncthreads();
+        cb = ce;
+    }
+
+    if (active) {
+        if (K::T00) out_s[t] = A.s00;
+        if (K::TIJ) {
+            out_t[t] = A.d0;
+            out_t[3 * n3v + t] = A.d1;
+            out_t[5 * n3v + t] = A.d2;
+            out_t[n3v + t] = A.o01;
+            out_t[2 * n3v + t] = A.o02;
+            out_t[4 * n3v + t] = A.o12;
+        }
+        if (K::T0I) {
+            out_t[t] = A.v0;
+            out_t[n3v + t] = A.v1;
+            out_t[2 * n3v + t] = A.v2;
+        }
+    }
+}
+
+template <int KIND, bool USE_PHI, bool ACCUM>
+void launch_t(gev_ctx *ctx, double *out_s, double *out_t, double a, double ms, double mt) {
+    const int N = ctx->N;
+    const int ntx = (N + TX - 1) / TX, nty = (N + TY - 1) / TY, ntz = (N + TZ - 1) / TZ;
+    const int64_t nblk = (int64_t)ntx * nty * ntz;
+    if ((N & (N - 1)) == 0)
+        hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, true>), dim3((unsigned)nblk), dim3(TPB), 0,
+                           ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, ctx->phi, out_s,
+                           out_t, N, ntx, nty, ntz, a, ms, mt);
+    else
+        hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, false>), dim3((unsigned)nblk), dim3(TPB), 0,
+                           ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, ctx->phi, out_s,
+                           out_t, N, ntx, nty, ntz, a, ms, mt);
+}
+
+template <int KIND>
+void launch_k(gev_ctx *ctx, bool use_phi, bool accum, double *out_s, double *out_t, double a, double ms,
+              double mt) {
+    if (use_phi) {
+        if (accum) launch_t<KIND, true, true>(ctx, out_s, out_t, a, ms, mt);
+        else launch_t<KIND, true, false>(ctx, out_s, out_t, a, ms, mt);
+    } else {
+        if (accum) launch_t<KIND, false, true>(ctx, out_s, out_t, a, ms, mt);
+        else launch_t<KIND, false, false>(ctx, out_s, out_t, a, ms, mt);
+    }
+}
+
+}  // namespace
+
+namespace gev {
+
+int launch_deposit(gev_ctx *ctx, int kind, double a, int use_phi, double coeff) {
+    const double dx = 1.0 / (double)ctx->N;
+    // mass factors of gevolution.hpp:945-947 / 1064-1065 / 1191-1193
+    double m = coeff / (dx * dx * dx);
+    m *= ctx->pmass;
+    const double m_a = m / a;
+    switch (kind) {
+    case DEP_T00: {
+        ScopedTimer t(ctx, "projection_T00_project");
+        launch_k<1>(ctx, use_phi, !ctx->zeroed[GEV_SOURCE], ctx->source, nullptr, a, m_a, m_a);
+        ctx->zeroed[GEV_SOURCE] = false;
+        break;
+    }
+    case DEP_TIJ: {
+        ScopedTimer t(ctx, "projection_Tij_project");
+        launch_k<2>(ctx, use_phi, !ctx->zeroed[GEV_SIJ], nullptr, ctx->Sij, a, m_a, m_a);
+        ctx->zeroed[GEV_SIJ] = false;
+        break;
+    }
+    case DEP_T00_TIJ: {
+        // a zeroed target reads back zeros when the other one must accumulate
+        ScopedTimer t(ctx, "projection_T00_Tij_project");
+        const bool acc = !ctx->zeroed[GEV_SOURCE] || !ctx->zeroed[GEV_SIJ];
+        launch_k<3>(ctx, true, acc, ctx->source, ctx->Sij, a, m_a, m_a);
+        ctx->zeroed[GEV_SOURCE] = false;
+        ctx->zeroed[GEV_SIJ] = false;
+        break;
+    }
+    case DEP_T0I: {
+        ScopedTimer t(ctx, "projection_T0i_project");
+        launch_k<4>(ctx, use_phi, !ctx->zeroed[GEV_BI], nullptr, ctx->Bi, 1.0, m, m);
+        ctx->zeroed[GEV_BI] = false;
+        break;
+    }
+    case DEP_CIC: {
+        // projection_T00_project with a = 1 and phi = NULL
+        ScopedTimer t(ctx, "scalar_projection_cic_project");
+        launch_k<5>(ctx, false, !ctx->zeroed[GEV_SOURCE], ctx->source, nullptr, 1.0, m, m);
+        ctx->zeroed[GEV_SOURCE] = false;
+        break;
+    }
+    default:
+        return fail(ctx, GEV_EINVAL, "unknown deposit kind");
+    }
+    return check_hip(ctx, hipGetLastError(), "deposit launch");
+}
+
+}  // namespace gev

@@ -191,12 +191,24 @@ __global__ __launch_bounds__(TPB) void k_r2c_x(const T *__restrict__ in,
     __shared__ C tw[M];
     const int64_t row0 = (int64_t)blockIdx.x * LINES;
     for (int t = threadIdx.x; t < M; t += TPB) tw[t] = twN[2 * t];
-    for (int idx = threadIdx.x; idx < LINES * M; idx += TPB) {
-        const int line = idx / M, n = idx % M;
-        const int64_t row = row0 + line;
-        C v = mk2<C>(0, 0);
-        if (row < nrows) v = reinterpret_cast<const C *>(in + row * N)[n];
-        buf[idx] = v;
+    {
+        // all loads of this thread in flight before the first LDS write
+        constexpr int PER = (LINES * M + TPB - 1) / TPB;
+        C v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int idx = threadIdx.x + k * TPB;
+            const int line = idx / M, n = idx % M;
+            const int64_t row = row0 + line;
+            v[k] = mk2<C>(0, 0);
+            if ((PER * TPB == LINES * M || idx < LINES * M) && row < nrows)
+                v[k] = reinterpret_cast<const C *>(in + row * N)[n];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int idx = threadIdx.x + k * TPB;
+            if (PER * TPB == LINES * M || idx < LINES * M) buf[idx] = v[k];
+        }
     }
     __syncthreads();
     stages<C, M, LINES, false, 1>(buf, tw);
@@ -226,23 +238,37 @@ __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *
     __shared__ C tw[M];
     const int64_t row0 = (int64_t)blockIdx.x * LINES;
     for (int t = threadIdx.x; t < M; t += TPB) tw[t] = twN[2 * t];
-    for (int idx = threadIdx.x; idx < LINES * M; idx += TPB) {
-        const int line = idx / M, k = idx % M;
-        const int64_t row = row0 + line;
-        C z = mk2<C>(0, 0);
-        if (row < nrows) {
-            C Xk = in[row * P + k];
-            C Xm = in[row * P + (M - k)];
+    {
+        // all loads of this thread in flight before the first LDS write
+        constexpr int PER = (LINES * M + TPB - 1) / TPB;
+        C xa[PER], xb[PER], wk[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int idx = threadIdx.x + j * TPB;
+            const int line = idx / M, k = idx % M;
+            const int64_t row = row0 + line;
+            xa[j] = xb[j] = wk[j] = mk2<C>(0, 0);
+            if ((PER * TPB == LINES * M || idx < LINES * M) && row < nrows) {
+                xa[j] = in[row * P + k];
+                xb[j] = in[row * P + (M - k)];
+                wk[j] = twN[k];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int idx = threadIdx.x + j * TPB;
+            if (!(PER * TPB == LINES * M || idx < LINES * M)) continue;
+            const int k = idx % M;
+            C Xk = xa[j], Xm = xb[j];
             if (k == 0) {
                 Xk.y = 0;
                 Xm.y = 0;
             }
             C S = cadd(Xk, cconj(Xm));
             C D = csub(Xk, cconj(Xm));
-            C Z = cadd(S, mul_pi(cmul(cconj(twN[k]), D)));
-            z = cconj(Z);  // inverse transform through the forward one
+            C Z = cadd(S, mul_pi(cmul(cconj(wk[j]), D)));
+            buf[idx] = cconj(Z);  // inverse transform through the forward one
         }
-        buf[idx] = z;
     }
     __syncthreads();
     stages<C, M, LINES, false, 1>(buf, tw);
@@ -276,11 +302,24 @@ __global__ __launch_bounds__(TPB) void k_c2c_cols(const typename cplx_of<T>::typ
     const int kx0 = blockIdx.x * W;
     const int64_t base = (int64_t)blockIdx.y * os + kx0;
     for (int t = threadIdx.x; t < N; t += TPB) tw[t] = twN[t];
-    for (int idx = threadIdx.x; idx < N * W; idx += TPB) {
-        const int w = idx % W, e = idx / W;
-        C v = mk2<C>(0, 0);
-        if (kx0 + w < H) v = in[base + e * es + w];
-        buf[idx] = INV ? cconj(v) : v;
+    {
+        // all loads of this thread in flight before the first LDS write; W
+        // divides TPB, so the column (and its bounds check) is fixed per thread
+        constexpr int PER = (N * W + TPB - 1) / TPB;
+        const int w = threadIdx.x % W;
+        const bool ok = kx0 + w < H;
+        C v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int idx = threadIdx.x + k * TPB;
+            v[k] = mk2<C>(0, 0);
+            if (ok && (PER * TPB == N * W || idx < N * W)) v[k] = in[base + (int64_t)(idx / W) * es + w];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int idx = threadIdx.x + k * TPB;
+            if (PER * TPB == N * W || idx < N * W) buf[idx] = INV ? cconj(v[k]) : v[k];
+        }
     }
     __syncthreads();
     stages<C, N, W, true, 1>(buf, tw);

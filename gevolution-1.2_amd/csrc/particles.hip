@@ -1,10 +1,5 @@
-// particles.hip -- particle-mesh kernels of one box:
-//   * CIC projections (gevolution.hpp:926-1297, LATfield2
-//     scalarProjectionCIC_project) as an owner-computes gather: one thread
-//     per target cell visits the (up to) 8 source cells whose particles
-//     deposit into it, in the order a sequential site loop would add them, and
-//     stores the sum once.  No atomics; the result is bit-identical to the
-//     reference's sequential scatter for the same in-cell particle order.
+// particles.hip -- particle kernels of one box (the CIC projections are in
+// deposit.hip):
 //   * re-filing (the cell bookkeeping of LATfield2 Particles::moveParticles):
 //     per-cell counts, CSR offsets by a three-phase scan, and a permutation
 //     listing each cell's particles in ascending ID.
@@ -21,166 +16,18 @@ __device__ __forceinline__ int64_t L(int N, int x, int y, int z) {
 }
 __device__ __forceinline__ int wrapi(int i, int N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
 
-// ------------------------------------------------------------ projections
-
-// KIND: 1 = T00, 2 = Tij, 3 = T00 + Tij, 4 = T0i, 5 = plain CIC (a = 1, phi = NULL)
-template <int KIND, bool USE_PHI, bool ACCUM>
-__global__ __launch_bounds__(TPB) void k_deposit(
-    const double *__restrict__ pos, const double *__restrict__ q, int64_t cap,
-    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
-    const double *__restrict__ phi, double *__restrict__ out_s, double *__restrict__ out_t, int N,
-    double a, double mass_s, double mass_t) {
-    constexpr bool DO_T00 = KIND == 1 || KIND == 3 || KIND == 5;
-    constexpr bool DO_TIJ = KIND == 2 || KIND == 3;
-    constexpr bool DO_T0I = KIND == 4;
-    const int64_t n3v = (int64_t)N * N * N;
-    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= n3v) return;
-    const int x = (int)(t % N);
-    const int y = (int)((t / N) % N);
-    const int z = (int)(t / ((int64_t)N * N));
-    const double dx = 1.0 / (double)N;
-    const int xp = x == N - 1 ? 0 : x + 1, yp = y == N - 1 ? 0 : y + 1, zp = z == N - 1 ? 0 : z + 1;
-
-    // phi at the target and its upper neighbours (zero without phi)
-    double ph = 0., phx = 0., phy = 0., phz = 0., phxy = 0., phxz = 0., phyz = 0.;
-    if (USE_PHI) {
-        ph = phi[t];
-        phx = phi[L(N, xp, y, z)];
-        phy = phi[L(N, x, yp, z)];
-        phz = phi[L(N, x, y, zp)];
-        phxy = phi[L(N, xp, yp, z)];
-        phxz = phi[L(N, xp, y, zp)];
-        phyz = phi[L(N, x, yp, zp)];
-    }
-    // plaquette sums in the reference's operand order (gevolution.hpp:1263-1272)
-    const double plq01 = ph + phy + phx + phxy;
-    const double plq02 = ph + phz + phx + phxz;
-    const double plq12 = ph + phz + phy + phyz;
-    // link factors of projection_T0i_project (gevolution.hpp:1129-1144)
-    const double lnk0 = 1. + ph + phx, lnk1 = 1. + ph + phy, lnk2 = 1. + ph + phz;
-
-    double acc00 = 0., acc_d[3] = {0., 0., 0.}, acc_o[3] = {0., 0., 0.}, acc_v[3] = {0., 0., 0.};
-    if (ACCUM) {
-        if (DO_T00) acc00 = out_s[t];
-        if (DO_TIJ) {
-            acc_d[0] = out_t[t];
-            acc_d[1] = out_t[3 * n3v + t];
-            acc_d[2] = out_t[5 * n3v + t];
-            acc_o[0] = out_t[n3v + t];
-            acc_o[1] = out_t[2 * n3v + t];
-            acc_o[2] = out_t[4 * n3v + t];
-        }
-        if (DO_T0I) {
-            acc_v[0] = out_t[t];
-            acc_v[1] = out_t[n3v + t];
-            acc_v[2] = out_t[2 * n3v + t];
-        }
-    }
-
-    const double *px = pos, *py = pos + cap, *pz = pos + 2 * cap;
-    const double *qx = q, *qy = q + cap, *qz = q + 2 * cap;
-
-    // Sources s = t - d, d in {0,1}^3, visited in increasing (wrapped) site
-    // index: per axis the lower neighbour first unless it wraps to N-1.
-#pragma unroll 1
-    for (int iz = 0; iz < 2; iz++) {
-        const int dz = z > 0 ? 1 - iz : iz;
-        const int sz = wrapi(z - dz, N);
-#pragma unroll 1
-        for (int iy = 0; iy < 2; iy++) {
-            const int dy = y > 0 ? 1 - iy : iy;
-            const int sy = wrapi(y - dy, N);
-#pragma unroll 1
-            for (int ix = 0; ix < 2; ix++) {
-                const int dxx = x > 0 ? 1 - ix : ix;
-                const int sx = wrapi(x - dxx, N);
-                // components this source feeds at t
-                const bool f_o01 = DO_TIJ && dxx == 0 && dy == 0;
-                const bool f_o02 = DO_TIJ && dxx == 0 && dz == 0;
-                const bool f_o12 = DO_TIJ && dy == 0 && dz == 0;
-                const bool f_v0 = DO_T0I && dxx == 0;
-                const bool f_v1 = DO_T0I && dy == 0;
-                const bool f_v2 = DO_T0I && dz == 0;
-                const int64_t s = L(N, sx, sy, sz);
-                const uint32_t b0 = start[s], b1 = start[s + 1];
-                if (b0 == b1) continue;
-                const double rx = sx * dx, ry = sy * dx, rz = sz * dx;
-                double lc = 0., ld[3] = {0., 0., 0.}, lo01 = 0., lo02 = 0., lo12 = 0.;
-                double lv0 = 0., lv1 = 0., lv2 = 0.;
-                for (uint32_t k = b0; k < b1; k++) {
-                    const uint32_t p = perm[k];
-                    const double up0 = (px[p] - rx) / dx, up1 = (py[p] - ry) / dx, up2 = (pz[p] - rz) / dx;
-                    const double dn0 = 1.0 - up0, dn1 = 1.0 - up1, dn2 = 1.0 - up2;
-                    const double W0 = dxx ? up0 : dn0, W1 = dy ? up1 : dn1, W2 = dz ? up2 : dn2;
-                    if (DO_T00) {
-                        double e = a, f = 0.;
-                        if (KIND != 5 && USE_PHI) {
-                            const double q0 = qx[p], q1 = qy[p], q2 = qz[p];
-                            f = q0 * q0 + q1 * q1 + q2 * q2;
-                            e = sqrt(f + a * a);
-                            f = 3. * e + f / e;
-                        }
-                        lc += W0 * W1 * W2 * (e + f * ph);
-                    }
-                    if (DO_TIJ) {
-                        const double q0 = qx[p], q1 = qy[p], q2 = qz[p];
-                        double f = q0 * q0 + q1 * q1 + q2 * q2;
-                        const double e = sqrt(f + a * a);
-                        f = 4. + a * a / (f + a * a);
-                        const double qq[3] = {q0, q1, q2};
-#pragma unroll
-                        for (int i = 0; i < 3; i++) {
-                            const double w = mass_t * qq[i] * qq[i] / e;
-                            ld[i] += w * W0 * W1 * W2 * (1. + f * ph);
-                        }
-                        if (f_o01) lo01 += mass_t * q0 * q1 / e * W2 * (1. + f * 0.25 * plq01);
-                        if (f_o02) lo02 += mass_t * q0 * q2 / e * W1 * (1. + f * 0.25 * plq02);
-                        if (f_o12) lo12 += mass_t * q1 * q2 / e * W0 * (1. + f * 0.25 * plq12);
-                    }
-                    if (DO_T0I) {
-                        if (f_v0) lv0 += mass_t * qx[p] * W1 * W2;
-                        if (f_v1) lv1 += mass_t * qy[p] * W0 * W2;
-                        if (f_v2) lv2 += mass_t * qz[p] * W0 * W1;
-                    }
-                }
-                if (DO_T00) acc00 += lc * mass_s;
-                if (DO_TIJ) {
-                    acc_d[0] += ld[0];
-                    acc_d[1] += ld[1];
-                    acc_d[2] += ld[2];
-                    if (f_o01) acc_o[0] += lo01;
-                    if (f_o02) acc_o[1] += lo02;
-                    if (f_o12) acc_o[2] += lo12;
-                }
-                if (DO_T0I) {
-                    if (f_v0) acc_v[0] += lv0 * lnk0;
-                    if (f_v1) acc_v[1] += lv1 * lnk1;
-                    if (f_v2) acc_v[2] += lv2 * lnk2;
-                }
-            }
-        }
-    }
-    if (DO_T00) out_s[t] = acc00;
-    if (DO_TIJ) {
-        out_t[t] = acc_d[0];
-        out_t[3 * n3v + t] = acc_d[1];
-        out_t[5 * n3v + t] = acc_d[2];
-        out_t[n3v + t] = acc_o[0];
-        out_t[2 * n3v + t] = acc_o[1];
-        out_t[4 * n3v + t] = acc_o[2];
-    }
-    if (DO_T0I) {
-        out_t[t] = acc_v[0];
-        out_t[n3v + t] = acc_v[1];
-        out_t[2 * n3v + t] = acc_v[2];
-    }
-}
-
 // ------------------------------------------------------------ filing
 
+// x / dx with dx = 1/N.  For N a power of two dx is exact and the quotient
+// equals x * N bit for bit, so the multiply replaces the division.
+template <bool POW2> __device__ __forceinline__ double over_dx(double x, double dx, double Nd) {
+    return POW2 ? x * Nd : x / dx;
+}
+
 // Cell of one coordinate: floor(p/dx) with a true division.
-__device__ __forceinline__ int cell1(double p, double dx) { return (int)floor(p / dx); }
+template <bool POW2> __device__ __forceinline__ int cell1(double p, double dx, double Nd) {
+    return (int)floor(over_dx<POW2>(p, dx, Nd));
+}
 
 __global__ __launch_bounds__(TPB) void k_count_cells(const double *__restrict__ pos, int64_t cap,
                                                      int64_t n, int N, uint32_t *__restrict__ count,
@@ -189,7 +36,8 @@ __global__ __launch_bounds__(TPB) void k_count_cells(const double *__restrict__ 
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= n) return;
     const double dx = 1.0 / (double)N;
-    const int cx = cell1(pos[p], dx), cy = cell1(pos[cap + p], dx), cz = cell1(pos[2 * cap + p], dx);
+    const int cx = cell1<false>(pos[p], dx, N), cy = cell1<false>(pos[cap + p], dx, N);
+    const int cz = cell1<false>(pos[2 * cap + p], dx, N);
     const uint32_t c = (uint32_t)L(N, cx, cy, cz);
     cell[p] = c;
     slot[p] = atomicAdd(count + c, 1u);
@@ -302,23 +150,25 @@ __global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__
 
 // ------------------------------------------------------------ kick / drift
 
-struct Nbr {
-    int x, y, z, xm, xp, ym, yp, zm, zp;
-};
+// Particles are updated tile by tile: a workgroup takes the particles filed
+// in a 16x4x4 block of cells (runs of the filing permutation) and first
+// stages phi, chi and B over the block plus a one-cell halo in LDS, so every
+// gather of update_q / update_pos is an LDS read.
+constexpr int KTX = 16, KTY = 4, KTZ = 4;
+constexpr int FX = KTX + 2, FY = KTY + 2, FZ = KTZ + 2;
+constexpr int FREG = FX * FY * FZ;  // 648 staged cells per field
 
+// one field around the particle's cell: (dx, dy, dz) in {-1, 0, 1}^3
 struct FieldView {
-    const double *f;
-    int N;
-    Nbr c;
+    const double *f;  // LDS image of the field over the staged region
+    int base;         // region index of the particle's cell
     __device__ __forceinline__ double operator()(int dx, int dy, int dz) const {
-        const int X = dx < 0 ? c.xm : (dx > 0 ? c.xp : c.x);
-        const int Y = dy < 0 ? c.ym : (dy > 0 ? c.yp : c.y);
-        const int Z = dz < 0 ? c.zm : (dz > 0 ? c.zp : c.z);
-        return f[L(N, X, Y, Z)];
+        return f[base + dx + FX * (dy + FY * dz)];
     }
 };
 
 // update_q, gevolution.hpp:570-678 (GRADIENT_ORDER 1); returns v^2/a^2
+template <bool POW2>
 __device__ __forceinline__ double update_q(double q[3], const double rd[3], FieldView phi,
                                            FieldView chi, FieldView B0, FieldView B1, FieldView B2,
                                            int nfield, const double params[2], double dtau,
@@ -391,13 +241,14 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], Fiel
     v2 = 0.;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        q[i] -= dtau * e2 * gradphi[i] / dx;
+        q[i] -= over_dx<POW2>(dtau * e2 * gradphi[i], dx, 1.0 / dx);
         v2 += q[i] * q[i];
     }
     return v2 / params[0] / params[0];
 }
 
 // update_q_Newton, gevolution.hpp:709-776
+template <bool POW2>
 __device__ __forceinline__ double update_q_newton(double q[3], const double rd[3], FieldView psi,
                                                   FieldView chi, int nfield, const double params[2],
                                                   double dtau, double dx) {
@@ -431,7 +282,7 @@ __device__ __forceinline__ double update_q_newton(double q[3], const double rd[3
     double v2 = 0.;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        q[i] -= dtau * params[0] * g[i] / dx;
+        q[i] -= over_dx<POW2>(dtau * params[0] * g[i], dx, 1.0 / dx);
         v2 += q[i] * q[i];
     }
     return v2 / params[0] / params[0];
@@ -494,12 +345,12 @@ __device__ __forceinline__ void update_pos(double pos[3], const double q[3], con
 
 // periodic wrap of the re-filing step; a coordinate whose cell would round
 // to N is set to 0 so that 0 <= floor(p/dx) < N for every filed particle
-__device__ __forceinline__ double wrap_pos(double p, double dx, int N) {
+template <bool POW2> __device__ __forceinline__ double wrap_pos(double p, double dx, int N) {
     if (p >= 1.0) p -= 1.0;
     if (p < 0.0) p += 1.0;
     // beyond one box length: reduce modulo 1; non-finite: file at 0
     if (!(p >= 0.0 && p < 1.0)) p = p - floor(p);
-    if (!(p >= 0.0) || (int)floor(p / dx) >= N) p = 0.0;
+    if (!(p >= 0.0) || cell1<POW2>(p, dx, N) >= N) p = 0.0;
     return p;
 }
 
@@ -509,52 +360,117 @@ struct StepArgs {
     double pk[2], pd[2];
 };
 
-// Per particle: locate, optional kick, optional drift (+wrap + count into
-// the new cell).  QM / PM select the callbacks (0 = GR, 1 = Newton).
-template <bool KICK, int QM, bool DRIFT, int PM>
-__global__ __launch_bounds__(TPB) void k_update(double *__restrict__ pos, double *__restrict__ q,
-                                                int64_t cap, int64_t n, int N,
-                                                const double *__restrict__ phi,
-                                                const double *__restrict__ chi,
-                                                const double *__restrict__ Bi, StepArgs sa,
-                                                unsigned long long *__restrict__ vmax,
-                                                uint32_t *__restrict__ count,
-                                                uint32_t *__restrict__ cell,
-                                                uint32_t *__restrict__ slot) {
-    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    double r = 0.;
-    if (p < n) {
-        const double dx = 1.0 / (double)N;
-        const int64_t n3v = (int64_t)N * N * N;
+// Per tile: stage the fields, then for every particle filed in the tile:
+// locate, optional kick, optional drift (+ wrap + count into the new cell).
+// QM / PM select the callbacks (0 = GR, 1 = Newton).  The filing
+// (start / perm) must describe the current positions.
+template <bool KICK, int QM, bool DRIFT, int PM, bool POW2>
+__global__ __launch_bounds__(TPB) void k_update_tiled(
+    double *__restrict__ pos, double *__restrict__ q, int64_t cap, int N, int ntx, int nty, int ntz,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const double *__restrict__ phi,
+    const double *__restrict__ chi, const double *__restrict__ Bi, StepArgs sa,
+    unsigned long long *__restrict__ vmax, uint32_t *__restrict__ count, uint32_t *__restrict__ cell,
+    uint32_t *__restrict__ slot) {
+    __shared__ double sf[5][FREG];
+    __shared__ uint32_t s_rowbeg[KTY * KTZ], s_rowoff[KTY * KTZ];
+    __shared__ unsigned long long sh[TPB / 64];
+
+    const int64_t nblk = (int64_t)ntx * nty * ntz;
+    int64_t tile = blockIdx.x;
+    if (nblk % 8 == 0) tile = (int64_t)(blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;  // XCD-aware
+    const int x0 = (int)(tile % ntx) * KTX, y0 = (int)((tile / ntx) % nty) * KTY;
+    const int z0 = (int)(tile / ((int64_t)ntx * nty)) * KTZ;
+    const int nx = min(KTX, N - x0), ny = min(KTY, N - y0), nz = min(KTZ, N - z0);
+    const int64_t n3v = (int64_t)N * N * N;
+    const double dx = 1.0 / (double)N;
+
+    const bool need_phi = KICK || (DRIFT && PM == 0 && sa.nfd >= 1);
+    const bool need_chi = (KICK && sa.nfk >= 2) || (DRIFT && PM == 0 && sa.nfd >= 2);
+    const bool need_B = (KICK && QM == 0 && sa.nfk >= 3) || (DRIFT && PM == 0 && sa.nfd >= 3);
+    // all staging loads of this thread in flight before the first LDS write
+    constexpr int PER = (FREG + TPB - 1) / TPB;
+    double v[5][PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = threadIdx.x + k * TPB;
+        if (i < FREG) {
+            const int rx = i % FX, ry = (i / FX) % FY, rz = i / (FX * FY);
+            const int64_t g = L(N, wrapi(x0 - 1 + rx, N), wrapi(y0 - 1 + ry, N), wrapi(z0 - 1 + rz, N));
+            if (need_phi) v[0][k] = phi[g];
+            if (need_chi) v[1][k] = chi[g];
+            if (need_B) {
+                v[2][k] = Bi[g];
+                v[3][k] = Bi[n3v + g];
+                v[4][k] = Bi[2 * n3v + g];
+            }
+        }
+    }
+    // rows of the tile: cells (x0 .. x0+nx-1, y, z) are one run of the permutation
+    if (threadIdx.x < KTY * KTZ) {
+        const int ly = threadIdx.x % KTY, lz = threadIdx.x / KTY;
+        uint32_t b = 0, e = 0;
+        if (ly < ny && lz < nz) {
+            const int64_t g0 = L(N, x0, y0 + ly, z0 + lz);
+            b = start[g0];
+            e = start[g0 + nx];
+        }
+        s_rowbeg[threadIdx.x] = b;
+        s_rowoff[threadIdx.x] = e - b;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = threadIdx.x + k * TPB;
+        if (i < FREG) {
+            if (need_phi) sf[0][i] = v[0][k];
+            if (need_chi) sf[1][i] = v[1][k];
+            if (need_B) {
+                sf[2][i] = v[2][k];
+                sf[3][i] = v[3][k];
+                sf[4][i] = v[4][k];
+            }
+        }
+    }
+    __syncthreads();
+    // row offsets (every thread scans the 16 row counts itself)
+    uint32_t rowoff[KTY * KTZ + 1];
+    rowoff[0] = 0;
+#pragma unroll
+    for (int r = 0; r < KTY * KTZ; r++) rowoff[r + 1] = rowoff[r] + s_rowoff[r];
+    const uint32_t total = rowoff[KTY * KTZ];
+
+    double rmax = 0.;
+    for (uint32_t j = threadIdx.x; j < total; j += TPB) {
+        int row = 0;
+        uint32_t roff = 0;
+#pragma unroll
+        for (int r = 1; r < KTY * KTZ; r++)
+            if (rowoff[r] <= j) {
+                row = r;
+                roff = rowoff[r];
+            }
+        const uint32_t p = perm[s_rowbeg[row] + (j - roff)];
         double x[3] = {pos[p], pos[cap + p], pos[2 * cap + p]};
         double qq[3] = {q[p], q[cap + p], q[2 * cap + p]};
-        // filed site and ref_dist = modf(pos/dx)
+        // filed site and ref_dist = modf(pos/dx) (Particles_gevolution.hpp:170 idiom)
         double rd[3];
         int ci[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double t = x[i] / dx;
+            const double t = over_dx<POW2>(x[i], dx, (double)N);
             const double fl = floor(t);
             ci[i] = (int)fl;
             rd[i] = t - fl;
         }
-        Nbr nb;
-        nb.x = ci[0];
-        nb.y = ci[1];
-        nb.z = ci[2];
-        nb.xm = nb.x == 0 ? N - 1 : nb.x - 1;
-        nb.xp = nb.x == N - 1 ? 0 : nb.x + 1;
-        nb.ym = nb.y == 0 ? N - 1 : nb.y - 1;
-        nb.yp = nb.y == N - 1 ? 0 : nb.y + 1;
-        nb.zm = nb.z == 0 ? N - 1 : nb.z - 1;
-        nb.zp = nb.z == N - 1 ? 0 : nb.z + 1;
-        FieldView fphi{phi, N, nb}, fchi{chi, N, nb};
-        FieldView fb0{Bi, N, nb}, fb1{Bi + n3v, N, nb}, fb2{Bi + 2 * n3v, N, nb};
+        const int base = (ci[0] - x0 + 1) + FX * ((ci[1] - y0 + 1) + FY * (ci[2] - z0 + 1));
+        const FieldView fphi{sf[0], base}, fchi{sf[1], base};
+        const FieldView fb0{sf[2], base}, fb1{sf[3], base}, fb2{sf[4], base};
         if (KICK) {
+            double r;
             if (QM == 0)
-                r = update_q(qq, rd, fphi, fchi, fb0, fb1, fb2, sa.nfk, sa.pk, sa.dtk, dx);
+                r = update_q<POW2>(qq, rd, fphi, fchi, fb0, fb1, fb2, sa.nfk, sa.pk, sa.dtk, dx);
             else
-                r = update_q_newton(qq, rd, fphi, fchi, sa.nfk, sa.pk, sa.dtk, dx);
+                r = update_q_newton<POW2>(qq, rd, fphi, fchi, sa.nfk, sa.pk, sa.dtk, dx);
+            rmax = r > rmax ? r : rmax;
             q[p] = qq[0];
             q[cap + p] = qq[1];
             q[2 * cap + p] = qq[2];
@@ -567,23 +483,23 @@ __global__ __launch_bounds__(TPB) void k_update(double *__restrict__ pos, double
                 for (int l = 0; l < 3; l++) x[l] += sa.dtd * qq[l] / sa.pd[0];  // gevolution.hpp:902
             }
 #pragma unroll
-            for (int l = 0; l < 3; l++) x[l] = wrap_pos(x[l], dx, N);
+            for (int l = 0; l < 3; l++) x[l] = wrap_pos<POW2>(x[l], dx, N);
             pos[p] = x[0];
             pos[cap + p] = x[1];
             pos[2 * cap + p] = x[2];
-            const uint32_t c = (uint32_t)L(N, cell1(x[0], dx), cell1(x[1], dx), cell1(x[2], dx));
+            const uint32_t c = (uint32_t)L(N, cell1<POW2>(x[0], dx, N), cell1<POW2>(x[1], dx, N),
+                                           cell1<POW2>(x[2], dx, N));
             cell[p] = c;
             slot[p] = atomicAdd(count + c, 1u);
         }
     }
     if (KICK) {
         // max of the callback returns (>= 0, so the bit patterns order as integers)
-        unsigned long long b = __double_as_longlong(r);
+        unsigned long long b = __double_as_longlong(rmax);
         for (int off = 32; off > 0; off >>= 1) {
             unsigned long long o = __shfl_xor(b, off);
             b = o > b ? o : b;
         }
-        __shared__ unsigned long long sh[TPB / 64];
         if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = b;
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -596,82 +512,22 @@ __global__ __launch_bounds__(TPB) void k_update(double *__restrict__ pos, double
 
 template <bool KICK, int QM, bool DRIFT, int PM>
 void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
-    hipLaunchKernelGGL((k_update<KICK, QM, DRIFT, PM>), dim3(gev::grid_for(ctx->n, TPB)), dim3(TPB),
-                       0, ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->n, ctx->N, ctx->phi,
-                       ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot);
-}
-
-template <int KIND, bool USE_PHI, bool ACCUM>
-void launch_dep_t(gev_ctx *ctx, double *out_s, double *out_t, double a, double ms, double mt) {
-    hipLaunchKernelGGL((k_deposit<KIND, USE_PHI, ACCUM>), dim3(gev::grid_for(gev::n3(ctx), TPB)),
-                       dim3(TPB), 0, ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm,
-                       ctx->phi, out_s, out_t, ctx->N, a, ms, mt);
-}
-
-template <int KIND>
-void launch_dep_k(gev_ctx *ctx, bool use_phi, bool accum, double *out_s, double *out_t, double a,
-                  double ms, double mt) {
-    if (use_phi) {
-        if (accum) launch_dep_t<KIND, true, true>(ctx, out_s, out_t, a, ms, mt);
-        else launch_dep_t<KIND, true, false>(ctx, out_s, out_t, a, ms, mt);
-    } else {
-        if (accum) launch_dep_t<KIND, false, true>(ctx, out_s, out_t, a, ms, mt);
-        else launch_dep_t<KIND, false, false>(ctx, out_s, out_t, a, ms, mt);
-    }
+    const int N = ctx->N;
+    const int ntx = (N + KTX - 1) / KTX, nty = (N + KTY - 1) / KTY, ntz = (N + KTZ - 1) / KTZ;
+    const int64_t nblk = (int64_t)ntx * nty * ntz;
+    if ((N & (N - 1)) == 0)
+        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, true>), dim3((unsigned)nblk), dim3(TPB), 0,
+                           ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
+                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot);
+    else
+        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, false>), dim3((unsigned)nblk), dim3(TPB), 0,
+                           ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
+                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot);
 }
 
 }  // namespace
 
 namespace gev {
-
-int launch_deposit(gev_ctx *ctx, int kind, double a, int use_phi, double coeff) {
-    const double dx = 1.0 / (double)ctx->N;
-    // mass factors of gevolution.hpp:945-947 / 1064-1065 / 1191-1193
-    double m = coeff / (dx * dx * dx);
-    m *= ctx->pmass;
-    double m_a = m / a;
-    switch (kind) {
-    case DEP_T00: {
-        ScopedTimer t(ctx, "projection_T00_project");
-        launch_dep_k<1>(ctx, use_phi, !ctx->zeroed[GEV_SOURCE], ctx->source, nullptr, a, m_a, m_a);
-        ctx->zeroed[GEV_SOURCE] = false;
-        break;
-    }
-    case DEP_TIJ: {
-        ScopedTimer t(ctx, "projection_Tij_project");
-        launch_dep_k<2>(ctx, use_phi, !ctx->zeroed[GEV_SIJ], nullptr, ctx->Sij, a, m_a, m_a);
-        ctx->zeroed[GEV_SIJ] = false;
-        break;
-    }
-    case DEP_T00_TIJ: {
-        ScopedTimer t(ctx, "projection_T00_Tij_project");
-        const bool acc = !ctx->zeroed[GEV_SOURCE] || !ctx->zeroed[GEV_SIJ];
-        if (acc && (ctx->zeroed[GEV_SOURCE] != ctx->zeroed[GEV_SIJ])) {
-            // mixed state: accumulate into both (the zeroed one reads zeros)
-        }
-        launch_dep_k<3>(ctx, true, acc, ctx->source, ctx->Sij, a, m_a, m_a);
-        ctx->zeroed[GEV_SOURCE] = false;
-        ctx->zeroed[GEV_SIJ] = false;
-        break;
-    }
-    case DEP_T0I: {
-        ScopedTimer t(ctx, "projection_T0i_project");
-        launch_dep_k<4>(ctx, use_phi, !ctx->zeroed[GEV_BI], nullptr, ctx->Bi, 1.0, m, m);
-        ctx->zeroed[GEV_BI] = false;
-        break;
-    }
-    case DEP_CIC: {
-        ScopedTimer t(ctx, "scalar_projection_cic_project");
-        // projection_T00_project with a = 1 and phi = NULL
-        launch_dep_k<5>(ctx, false, !ctx->zeroed[GEV_SOURCE], ctx->source, nullptr, 1.0, m, m);
-        ctx->zeroed[GEV_SOURCE] = false;
-        break;
-    }
-    default:
-        return fail(ctx, GEV_EINVAL, "unknown deposit kind");
-    }
-    return check_hip(ctx, hipGetLastError(), "deposit launch");
-}
 
 // scan count[] into start[] and build perm[] from cell[] / slot[]
 static int finish_filing(gev_ctx *ctx) {
