@@ -21,11 +21,29 @@
 
 namespace {
 
-constexpr int TPB = 256;
-constexpr int TX = 16, TY = 4, TZ = 4;
+// Tile shape (compile-time; GEV_DEP_TX/TY/TZ/CAP override the defaults).
+#ifndef GEV_DEP_TX
+#define GEV_DEP_TX 16
+#endif
+#ifndef GEV_DEP_TY
+#define GEV_DEP_TY 4
+#endif
+#ifndef GEV_DEP_TZ
+#define GEV_DEP_TZ 4
+#endif
+#ifndef GEV_DEP_CAP
+#define GEV_DEP_CAP 448
+#endif
+#ifndef GEV_DEP_XCD
+#define GEV_DEP_XCD 1
+#endif
+constexpr int TX = GEV_DEP_TX, TY = GEV_DEP_TY, TZ = GEV_DEP_TZ;
+constexpr int TPB = TX * TY * TZ;   // one target cell per thread
 constexpr int SX = TX + 1, SY = TY + 1, SZ = TZ + 1;
-constexpr int NSRC = SX * SY * SZ;  // 425 source cells
-constexpr int CAP = 448;            // particle records per chunk (3 workgroups per CU)
+constexpr int NSRC = SX * SY * SZ;  // source cells of a tile (425 for 16x4x4)
+constexpr int CAP = GEV_DEP_CAP;    // particle records per chunk
+static_assert(NSRC <= 2 * TPB, "two source cells per thread");
+static_assert(TPB % 64 == 0 && TPB <= 1024, "whole wavefronts");
 
 // cell index; N^3 <= 2^30 fits 32 bits
 __device__ __forceinline__ uint32_t L(int N, int x, int y, int z) {
@@ -236,7 +254,9 @@ template <int KIND> __device__ __forceinline__ bool feeds(int dxx, int dy, int d
     return true;
 }
 
-template <int KIND, bool USE_PHI, bool ACCUM, bool POW2>
+// FULL: N is a multiple of the tile, so the tile and source-region extents
+// are compile-time constants (index arithmetic without integer division).
+template <int KIND, bool USE_PHI, bool ACCUM, bool POW2, bool FULL>
 __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const double *__restrict__ pos, const double *__restrict__ q, int64_t cap,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
@@ -247,51 +267,77 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     __shared__ uint32_t s_off[NSRC + 1];
     __shared__ uint32_t s_gbeg[NSRC];
     __shared__ uint32_t s_cnt[NSRC];
-    __shared__ uint32_t s_scan[TPB];
+    __shared__ uint32_t s_scan[TPB / 64];
     __shared__ uint16_t s_rc[CAP];
 
     // XCD-aware tile order: consecutive workgroups of one XCD (dispatched
     // round-robin over the 8 XCDs) take neighbouring tiles and share L2
     const uint32_t nblk = (uint32_t)ntx * nty * ntz;
     uint32_t tile = blockIdx.x;
-    if (nblk % 8 == 0) tile = (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;
+    if (GEV_DEP_XCD && nblk % 8 == 0) tile = (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;
     const int tx = (int)(tile % ntx), ty = (int)((tile / ntx) % nty), tz = (int)(tile / ((uint32_t)ntx * nty));
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-    const int nx = min(TX, N - x0), ny = min(TY, N - y0), nz = min(TZ, N - z0);
+    const int nx = FULL ? TX : min(TX, N - x0), ny = FULL ? TY : min(TY, N - y0);
+    const int nz = FULL ? TZ : min(TZ, N - z0);
     const int nsx = nx + 1, nsy = ny + 1, nsz = nz + 1;
     const int nsrc = nsx * nsy * nsz;
     const double dx = 1.0 / (double)N, Nd = (double)N;
     const int64_t n3v = (int64_t)N * N * N;
 
-    // 1. source cells in ascending site order: CSR begin and count
-    for (int c = threadIdx.x; c < nsrc; c += TPB) {
-        const int px = c % nsx, py = (c / nsx) % nsy, pz = c / (nsx * nsy);
-        const int gx = wrapi(x0 - 1 + order_slot(px, x0, nsx), N);
-        const int gy = wrapi(y0 - 1 + order_slot(py, y0, nsy), N);
-        const int gz = wrapi(z0 - 1 + order_slot(pz, z0, nsz), N);
-        const uint32_t g = L(N, gx, gy, gz);
-        const uint32_t b = start[g];
-        s_gbeg[c] = b;
-        s_cnt[c] = start[g + 1] - b;
+    // 1. source cells in ascending site order: CSR begin and count (two
+    //    cells per thread, all four loads in flight together)
+    {
+        uint32_t g[2], b[2], e[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = threadIdx.x + h * TPB;
+            g[h] = 0;
+            if (c < nsrc) {
+                const int px = c % nsx, py = (c / nsx) % nsy, pz = c / (nsx * nsy);
+                const int gx = wrapi(x0 - 1 + order_slot(px, x0, nsx), N);
+                const int gy = wrapi(y0 - 1 + order_slot(py, y0, nsy), N);
+                const int gz = wrapi(z0 - 1 + order_slot(pz, z0, nsz), N);
+                g[h] = L(N, gx, gy, gz);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            b[h] = start[g[h]];
+            e[h] = start[g[h] + 1];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = threadIdx.x + h * TPB;
+            if (c < nsrc) {
+                s_gbeg[c] = b[h];
+                s_cnt[c] = e[h] - b[h];
+            }
+        }
     }
     __syncthreads();
-    // exclusive scan of the counts (two cells per thread)
+    // exclusive scan of the counts, two consecutive cells per thread:
+    // shuffle scan inside each wavefront, wavefront totals through LDS
     {
         const int c = 2 * threadIdx.x;
         const uint32_t c0 = c < nsrc ? s_cnt[c] : 0u, c1 = c + 1 < nsrc ? s_cnt[c + 1] : 0u;
         const uint32_t v = c0 + c1;
-        s_scan[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < TPB; off <<= 1) {
-            const uint32_t u = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0u;
-            __syncthreads();
-            s_scan[threadIdx.x] += u;
-            __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        uint32_t incl = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(incl, off);
+            if (lane >= off) incl += u;
         }
-        const uint32_t excl = s_scan[threadIdx.x] - v;
+        if (lane == 63) s_scan[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; w++)
+            if (w < wave) before += s_scan[w];
+        const uint32_t excl = before + incl - v;
         if (c < nsrc) s_off[c] = excl;
         if (c + 1 < nsrc) s_off[c + 1] = excl + c0;
-        if (threadIdx.x == TPB - 1) s_off[nsrc] = s_scan[TPB - 1];
+        if (threadIdx.x == TPB - 1) s_off[nsrc] = before + incl;
         __syncthreads();
     }
 
@@ -474,14 +520,15 @@ void launch_t(gev_ctx *ctx, double *out_s, double *out_t, double a, double ms, d
     const int N = ctx->N;
     const int ntx = (N + TX - 1) / TX, nty = (N + TY - 1) / TY, ntz = (N + TZ - 1) / TZ;
     const int64_t nblk = (int64_t)ntx * nty * ntz;
-    if ((N & (N - 1)) == 0)
-        hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, true>), dim3((unsigned)nblk), dim3(TPB), 0,
-                           ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, ctx->phi, out_s,
-                           out_t, N, ntx, nty, ntz, a, ms, mt);
-    else
-        hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, false>), dim3((unsigned)nblk), dim3(TPB), 0,
-                           ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, ctx->phi, out_s,
-                           out_t, N, ntx, nty, ntz, a, ms, mt);
+    const bool pow2 = (N & (N - 1)) == 0, full = N % TX == 0 && N % TY == 0 && N % TZ == 0;
+#define GEV_DEP_LAUNCH(P2, FL)                                                                              \
+    hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, P2, FL>), dim3((unsigned)nblk), dim3(TPB), 0, \
+                       ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, ctx->phi, out_s, out_t, \
+                       N, ntx, nty, ntz, a, ms, mt)
+    if (pow2 && full) GEV_DEP_LAUNCH(true, true);
+    else if (full) GEV_DEP_LAUNCH(false, true);
+    else GEV_DEP_LAUNCH(false, false);
+#undef GEV_DEP_LAUNCH
 }
 
 template <int KIND>
