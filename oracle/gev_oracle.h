@@ -143,6 +143,47 @@ void orc_box_free(orc_box *b);
 void orc_cycle(const orc_sim *sim, orc_box *b, orc_state *st);
 double orc_next_dtau(const orc_sim *sim, double a);
 
+/* ---- initial conditions (ic_basic.hpp, prng_engine.hpp): gev_oracle_ic.c -- */
+/* counter-based PRNG: block function and the pure stream position -> output */
+void orc_prng_block(uint32_t seed, uint64_t counter, uint64_t out[4]);
+uint32_t orc_prng_at(uint32_t seed, uint64_t pos);
+/* natural cubic spline: c[n] = half second derivatives */
+int orc_spline_init(int n, const double *x, const double *y, double *c);
+double orc_spline_eval(int n, const double *x, const double *y, const double *c, double xq);
+/* generateCICKernel into a real field (numpcl = 0: the standard kernel) */
+void orc_generate_cic_kernel(int N, double *ker, int64_t numpcl, const float *pcldata, int numtile);
+/* generateDisplacementField (potFT: kernel in, realisation out) */
+void orc_generate_displacement_field(int N, double *potFT, double coeff, int nsp,
+                                     const double *spx, const double *spy, const double *spc,
+                                     uint32_t seed, int ksphere, int deconvolve_f,
+                                     int ignorekernel, int cr_log);
+void orc_initialize_particle_positions(int64_t numpart, const float *partdata, int numtile,
+                                       double *pos, int64_t *id);
+double orc_displace_pcls_ic(int N, int64_t n, double *pos, double coeff, const double *field);
+double orc_initialize_q_ic(int N, int64_t n, const double *pos, double *q, double coeff,
+                           const double *field);
+
+typedef struct {
+    uint32_t seed;
+    int ksphere;              /* k-domain = sphere */
+    int correct_displacement; /* template-aware CIC kernel */
+    int numtile;              /* tiling factor */
+    int cr_log;               /* 1: correctly rounded log(r1) instead of logf */
+    double z_in, A_s, n_s, k_pivot, boxsize;
+} orc_ic;
+
+double orc_pk_primordial(double k, const orc_ic *ic);
+/* spline ordinates for phi, CDM displacement and CDM velocity potential */
+void orc_ic_tables(const orc_sim *sim, const orc_ic *ic, int n, const double *k, const double *d_tot,
+                   const double *t_tot, const double *d_cdm, const double *t_cdm, double *y_phi,
+                   double *y_disp, double *y_vel);
+/* generateIC_basic, transfer-function path, one CDM species; b allocated for
+   numpcl * numtile^3 particles.  maxvel as main.cpp:326-330 leaves it. */
+int orc_generate_ic_basic(const orc_sim *sim, const orc_ic *ic, int64_t numpcl, const float *pcldata,
+                          int ntk, const double *k, const double *d_tot, const double *t_tot,
+                          const double *d_cdm, const double *t_cdm, orc_box *b, double *maxvel,
+                          double *max_disp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,26 @@ def lib():
         L.orc_box_free.argtypes = [_p]
         L.orc_cycle.argtypes = [_p, _p, _p]
         L.orc_set_threads.argtypes = [_i]
+        _u32, _u64 = ctypes.c_uint32, ctypes.c_uint64
+        L.orc_prng_block.argtypes = [_u32, _u64, _p]
+        L.orc_prng_at.restype = _u32
+        L.orc_prng_at.argtypes = [_u32, _u64]
+        L.orc_spline_init.restype = _i
+        L.orc_spline_init.argtypes = [_i, _p, _p, _p]
+        L.orc_spline_eval.restype = _d
+        L.orc_spline_eval.argtypes = [_i, _p, _p, _p, _d]
+        L.orc_generate_cic_kernel.argtypes = [_i, _p, _i64, _p, _i]
+        L.orc_generate_displacement_field.argtypes = [_i, _p, _d, _i, _p, _p, _p, _u32, _i, _i, _i, _i]
+        L.orc_initialize_particle_positions.argtypes = [_i64, _p, _i, _p, _p]
+        L.orc_displace_pcls_ic.restype = _d
+        L.orc_displace_pcls_ic.argtypes = [_i, _i64, _p, _d, _p]
+        L.orc_initialize_q_ic.restype = _d
+        L.orc_initialize_q_ic.argtypes = [_i, _i64, _p, _p, _d, _p]
+        L.orc_pk_primordial.restype = _d
+        L.orc_pk_primordial.argtypes = [_d, _p]
+        L.orc_ic_tables.argtypes = [_p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
+        L.orc_generate_ic_basic.restype = _i
+        L.orc_generate_ic_basic.argtypes = [_p, _p, _i64, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -96,6 +116,12 @@ class Box(ctypes.Structure):
                 ("phi", _p), ("chi", _p), ("source", _p), ("Sij", _p), ("Bi", _p),
                 ("scalarFT", _p), ("SijFT", _p), ("BiFT", _p),
                 ("cell", _p), ("count", _p), ("start", _p), ("perm", _p)]
+
+
+class IC(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint32), ("ksphere", _i), ("correct_displacement", _i),
+                ("numtile", _i), ("cr_log", _i), ("z_in", _d), ("A_s", _d), ("n_s", _d),
+                ("k_pivot", _d), ("boxsize", _d)]
 
 
 def make_cosmo(Omega_cdm=0.26, Omega_b=0.05, h=0.67, Omega_rad=0.0):
@@ -279,6 +305,128 @@ def move_particles(N, pos, q, method, dtau, phi, chi, Bi, nfield, params):
     return posn
 
 
+# ---------------------------------------------------------------- initial conditions
+
+def prng_block(seed, counter):
+    out = np.zeros(4, dtype=np.uint64)
+    lib().orc_prng_block(seed, counter, _ptr(out))
+    return out
+
+
+def prng_at(seed, pos):
+    return lib().orc_prng_at(seed, pos)
+
+
+def spline_init(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    c = np.zeros_like(x)
+    if lib().orc_spline_init(len(x), _ptr(x), _ptr(y), _ptr(c)) != 0:
+        raise ValueError("spline needs at least two knots")
+    return c
+
+
+def spline_eval(x, y, c, xq):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    return np.array([lib().orc_spline_eval(len(x), _ptr(x), _ptr(y), _ptr(c), float(v))
+                     for v in np.atleast_1d(xq)])
+
+
+def generate_cic_kernel(N, pcldata=None, numtile=1):
+    ker = np.zeros((N, N, N), dtype=np.float64)
+    if pcldata is None:
+        lib().orc_generate_cic_kernel(N, _ptr(ker), 0, None, 1)
+    else:
+        p = np.ascontiguousarray(pcldata, dtype=np.float32)
+        lib().orc_generate_cic_kernel(N, _ptr(ker), len(p), _ptr(p), numtile)
+    return ker
+
+
+def generate_displacement_field(N, potFT, coeff, x, y, c, seed, ksphere, deconvolve_f=1,
+                                ignorekernel=0, cr_log=0):
+    """Returns the realisation; potFT (the transformed kernel) is not modified."""
+    out = np.array(potFT, dtype=np.complex128, order="C", copy=True)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    lib().orc_generate_displacement_field(N, _ptr(out), coeff, len(x), _ptr(x), _ptr(y), _ptr(c),
+                                          seed, ksphere, deconvolve_f, ignorekernel, cr_log)
+    return out
+
+
+def initialize_particle_positions(pcldata, numtile):
+    p = np.ascontiguousarray(pcldata, dtype=np.float32)
+    n = len(p) * numtile ** 3
+    pos = np.zeros((n, 3), dtype=np.float64)
+    ids = np.zeros(n, dtype=np.int64)
+    lib().orc_initialize_particle_positions(len(p), _ptr(p), numtile, _ptr(pos), _ptr(ids))
+    return pos, ids
+
+
+def displace_pcls_ic(N, pos, coeff, field):
+    """In place on pos; returns the maximum displacement."""
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    return lib().orc_displace_pcls_ic(N, len(pos), _ptr(pos), coeff, _ptr(field))
+
+
+def initialize_q_ic(N, pos, coeff, field):
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    q = np.zeros_like(pos)
+    vmax = lib().orc_initialize_q_ic(N, len(pos), _ptr(pos), _ptr(q), coeff, _ptr(field))
+    return q, vmax
+
+
+def pk_primordial(k, ic):
+    return lib().orc_pk_primordial(k, ctypes.byref(ic))
+
+
+def ic_tables(sim, ic, k, d_tot, t_tot, d_cdm, t_cdm):
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (k, d_tot, t_tot, d_cdm, t_cdm)]
+    n = len(arrs[0])
+    outs = [np.zeros(n) for _ in range(3)]
+    lib().orc_ic_tables(ctypes.byref(sim), ctypes.byref(ic), n, *[_ptr(a) for a in arrs],
+                        *[_ptr(o) for o in outs])
+    return outs
+
+
+def load_template(path):
+    """Gadget-2 homogeneous template as loadHomogeneousTemplate returns it
+    (ic_basic.hpp:191-324): type-1 positions / BoxSize, float32 (n, 3)."""
+    raw = open(path, "rb").read()
+    b1 = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    if b1 != 256:
+        raise ValueError("template header not recognised")
+    npart = np.frombuffer(raw, np.uint32, 6, 4)
+    box = float(np.frombuffer(raw, np.float64, 1, 4 + 128)[0])
+    off = 4 + 256 + 4 + 4 + 12 * int(npart[0])
+    pos = np.frombuffer(raw, np.float32, 3 * int(npart[1]), off).reshape(-1, 3)
+    return (pos / np.float32(box)).astype(np.float32)
+
+
+def load_transfer_functions(path, qname, boxsize, h):
+    """CLASS-format table as loadTransferFunctions reads it (ic_basic.hpp:494-717):
+    columns found by their header labels; returns k*boxsize, d_<q>, t_<q>*boxsize/h."""
+    kcol = dcol = tcol = None
+    rows = []
+    for line in open(path):
+        if line.startswith("#"):
+            labels = line.split(":")[1:]
+            for i, lab in enumerate(labels):
+                if lab.startswith("k"):
+                    kcol = i
+                elif lab.startswith("d_" + qname):
+                    dcol = i
+                elif lab.startswith("t_" + qname):
+                    tcol = i
+            continue
+        if line.strip():
+            rows.append([float(v) for v in line.split()])
+    a = np.array(rows)
+    return a[:, kcol] * boxsize, a[:, dcol].copy(), a[:, tcol] * boxsize / h
+
+
 class OracleBox:
     """Owns an orc_box and exposes its arrays as numpy views."""
 
@@ -314,6 +462,18 @@ class OracleBox:
     def file(self):
         lib().orc_file_particles(self.N, self.n, self.box.pos, self.box.id, self.box.cell,
                                  self.box.count, self.box.start, self.box.perm)
+
+    def generate_ic(self, sim, ic, pcldata, k, d_tot, t_tot, d_cdm, t_cdm):
+        """generateIC_basic (transfer-function path); returns (maxvel, max_displacement)."""
+        p = np.ascontiguousarray(pcldata, dtype=np.float32)
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (k, d_tot, t_tot, d_cdm, t_cdm)]
+        mv, md = _d(0.), _d(0.)
+        rc = lib().orc_generate_ic_basic(ctypes.byref(sim), ctypes.byref(ic), len(p), _ptr(p), len(arrs[0]),
+                                         *[_ptr(a) for a in arrs], ctypes.byref(self.box), ctypes.byref(mv),
+                                         ctypes.byref(md))
+        if rc != 0:
+            raise ValueError("orc_generate_ic_basic failed")
+        return mv.value, md.value
 
     def cycle(self, sim, state):
         lib().orc_cycle(ctypes.byref(sim), ctypes.byref(self.box), ctypes.byref(state))
