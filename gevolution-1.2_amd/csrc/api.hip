@@ -149,7 +149,6 @@ static void rk4bg(double &a, double fourpiG, const gev_cosmology *c, double dtau
 
 using namespace gev;
 
-#define GEV_EXPORT extern "C" __attribute__((visibility("default")))
 
 GEV_EXPORT int64_t gev_ctx_footprint_bytes(const gev_box_desc *d) {
     if (!d || d->ngrid < 2 || d->npart < 0) return -1;
@@ -575,19 +574,22 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st) {
         rc = launch_field_sum(c, c->source, n3(c), nullptr);  // T00hom * N^3 -> red_d[0]
         if (rc) return rc;
         if (st->dtau_old > 0.) {
+            // prepareFTsource inside the first FFT pass, solveModifiedPoissonFT
+            // inside the z passes (main.cpp:472-488); only phi(k=0) of scalarFT is kept
             const double H = hconf(a, fourpiG, cs);
-            rc = launch_prepare_scalar(c, cs->Omega_cdm + cs->Omega_b + 0.0,
-                                       3. * H * dx * dx / st->dtau_old, fourpiG * dx * dx / a,
-                                       3. * H * H * dx * dx);
-            if (!rc) rc = fft_forward(c, c->source, c->scalarFT, 1);
-            if (!rc) rc = launch_poisson(c, 1. / (dx * dx), 3. * hconf(a, fourpiG, cs) / st->dtau_old);
-            if (!rc) rc = fft_backward(c, c->scalarFT, c->phi, 1);
+            rc = launch_x_prep_scalar(c, cs->Omega_cdm + cs->Omega_b + 0.0, 3. * H * dx * dx / st->dtau_old,
+                                      fourpiG * dx * dx / a, 3. * H * H * dx * dx);
+            if (!rc) rc = fft_forward_y(c, c->scalarFT, 1);
+            if (!rc)
+                rc = launch_zsolve_poisson(c, c->scalarFT, c->fft_tmp, 1. / (dx * dx),
+                                           3. * hconf(a, fourpiG, cs) / st->dtau_old, nullptr, c->scalarFT);
+            if (!rc) rc = fft_backward_yx(c, c->fft_tmp, c->phi, 1);
             if (rc) return rc;
         }
     } else {
-        rc = fft_forward(c, c->source, c->scalarFT, 1);
-        if (!rc) rc = launch_poisson(c, fourpiG / a, 0.);
-        if (!rc) rc = fft_backward(c, c->scalarFT, c->phi, 1);
+        rc = fft_forward_xy(c, c->source, c->scalarFT, 1);
+        if (!rc) rc = launch_zsolve_poisson(c, c->scalarFT, c->fft_tmp, fourpiG / a, 0., nullptr, c->scalarFT);
+        if (!rc) rc = fft_backward_yx(c, c->fft_tmp, c->phi, 1);
         if (rc) return rc;
     }
     // phi(k=0) for the background record (main.cpp:533)
@@ -595,21 +597,27 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st) {
                                      hipMemcpyDeviceToDevice, c->stream), "phi_k0");
     if (rc) return rc;
 
-    rc = launch_prepare_tensor(c, 2. * fourpiG * dx * dx / a);
-    if (!rc) rc = fft_forward(c, c->Sij, c->SijFT, 6);
+    // prepareFTsource(Sij) inside the first FFT pass, projectFTscalar -> chi
+    // and evolveFTvector -> BiFT inside the z passes (main.cpp:539-593).  SijFT is scratch
+    // after the solve: the z-inverted chi goes to its component 0 and B to
+    // components 1-3; scalarFT keeps only chi(k=0); BiFT, the carried state,
+    // is written in full.
+    double2 *const zout_chi = c->SijFT, *const zout_b = c->SijFT + nk(c);
+    rc = launch_x_prep_tensor(c, 2. * fourpiG * dx * dx / a);
+    if (!rc) rc = fft_forward_y(c, c->SijFT, 6);
     if (rc) return rc;
     if (p->vector_flag == 1) {
-        rc = launch_project_scalar(c, 0);
-        if (!rc) rc = fft_backward(c, c->scalarFT, c->chi, 1);
-        if (!rc) rc = fft_forward(c, c->Bi, c->BiFT, 3);
-        if (!rc) rc = launch_project_vector(c, fourpiG * dx * dx, 0.);
+        rc = launch_zsolve_tensor(c, c->SijFT, zout_chi, 0., false, nullptr, c->scalarFT);
+        if (!rc) rc = fft_backward_yx(c, zout_chi, c->chi, 1);
+        if (!rc) rc = fft_forward_xy(c, c->Bi, c->BiFT, 3);
+        if (!rc) rc = launch_zsolve_vector(c, c->BiFT, zout_b, fourpiG * dx * dx, 0., c->BiFT);
     } else {
-        rc = launch_project_scalar_evolve_vector(c, a * a * st->dtau_old);
-        if (!rc) rc = fft_backward(c, c->scalarFT, c->chi, 1);
+        rc = launch_zsolve_tensor(c, c->SijFT, zout_chi, a * a * st->dtau_old, true, nullptr, c->scalarFT);
+        if (!rc) rc = fft_backward_yx(c, zout_chi, c->chi, 1);
     }
     if (rc) return rc;
     if (p->gr_flag > 0) {
-        rc = fft_backward(c, c->BiFT, c->Bi, 3);
+        rc = fft_backward_yx(c, zout_b, c->Bi, 3);
         if (rc) return rc;
     }
     c->zeroed[GEV_PHI] = c->zeroed[GEV_CHI] = c->zeroed[GEV_BI] = false;

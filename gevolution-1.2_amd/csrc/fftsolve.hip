@@ -1,0 +1,393 @@
+// fftsolve.hip -- FFT passes fused with the work around them in a cycle.
+//
+// First pass: r2c along x applies prepareFTsource (gevolution.hpp:56-192) to
+// the rows it stages, so the prepared source is never stored (main.cpp:472-477,
+// 539-544).
+//
+// Last pass: the c2c along z of the forward transform fused with the
+// Fourier-space solves of gevolution.hpp and the first pass of the backward
+// transform (c2c along z, inverse), for the cycle sequences of main.cpp:
+//   source -> solveModifiedPoissonFT -> phi                  (main.cpp:477-488)
+//   Sij    -> projectFTscalar + evolveFTvector -> chi, B     (main.cpp:544-593)
+//   Sij    -> projectFTscalar -> chi                         (main.cpp:558-563)
+//   Bi     -> projectFTvector -> B                           (main.cpp:575-593)
+// A workgroup owns W consecutive kx columns at one ky.  It transforms each
+// input component's columns along z in LDS and keeps the results in
+// registers (PTS points per thread), evaluates the solve per k point with the
+// same per-point bodies as the standalone kernels (ksolve.h), and runs the
+// inverse z transform of every output component before one store.  Each
+// component column is read once and written once; the k-space intermediate
+// of the unfused sequence (z pass out, solve in/out, z-inverse in) never
+// reaches HBM.  The butterflies are those of fft.hip, so the fused result
+// equals the unfused one bit for bit.
+#include "fft_core.h"
+#include "gev_internal.h"
+#include "ksolve.h"
+#include "prep.h"
+
+namespace {
+
+using namespace gev_fft;
+using namespace gev_ks;
+using namespace gev_prep;
+
+enum { ZS_POISSON = 0, ZS_TENSOR = 1, ZS_TENSOR_SCALAR = 2, ZS_VECTOR = 3 };
+
+template <int KIND> struct ZSolve;
+template <> struct ZSolve<ZS_POISSON> { static constexpr int NIN = 1, NOUT = 1; };
+template <> struct ZSolve<ZS_TENSOR> { static constexpr int NIN = 6, NOUT = 4; };
+template <> struct ZSolve<ZS_TENSOR_SCALAR> { static constexpr int NIN = 6, NOUT = 1; };
+template <> struct ZSolve<ZS_VECTOR> { static constexpr int NIN = 3, NOUT = 3; };
+
+// columns per workgroup and workgroup size: 64-byte column rows (128 at
+// N >= 1024 would need 2x the registers), and at most 4 points per thread
+template <int N> constexpr int zs_cols() { return N >= 1024 ? 2 : 4; }
+template <int N> constexpr int zs_threads() {
+    int t = N * zs_cols<N>() / 4;
+    if (t > 512) t = 512;
+    if (t < 64) t = 64;
+    return (t + 63) / 64 * 64;
+}
+
+struct ZArgs {
+    const double2 *in;   // NIN components, after the x and y passes
+    double2 *out;        // NOUT components, after the inverse z pass
+    double2 *bift;       // B state: read (tensor) / written (tensor, vector)
+    double2 *kstore;     // optional: solved k values of output 0 (full field)
+    double2 *k0;         // optional: output 0 at k = 0 (one element)
+    const double *g;     // gridk2 per axis index
+    const double2 *ks;   // kshift per axis index
+    const double2 *tw;   // exp(-2 pi i t / N), t < N
+    int64_t nkv;         // complexes per component
+    int P, H;
+    double c0, c1;       // Poisson: coeff/(-N^3), modif; tensor: a^2 dtau; vector: coeff, modif
+};
+
+template <int N, int KIND>
+__global__ __launch_bounds__(zs_threads<N>()) void k_zsolve(ZArgs A) {
+    constexpr int W = zs_cols<N>();
+    constexpr int NT = zs_threads<N>();
+    constexpr int NIN = ZSolve<KIND>::NIN, NOUT = ZSolve<KIND>::NOUT;
+    constexpr int PTS = (N * W + NT - 1) / NT;
+    constexpr bool EXACT = PTS * NT == N * W;
+    static_assert(NT % W == 0, "a thread keeps one column");
+    __shared__ double2 buf[N * W];
+    __shared__ double2 tw[N];
+
+    // XCD-aware tile order: hardware workgroup ids go round-robin over the 8
+    // XCDs, so tiles l and l+1 (which share 128-byte rows) are given to ids
+    // 8 apart, i.e. to consecutive workgroups of one XCD (same L2)
+    const int ntx = gridDim.x;
+    const int64_t T = (int64_t)ntx * gridDim.y;
+    int64_t L = (int64_t)blockIdx.y * ntx + blockIdx.x;
+    if (T % 8 == 0) L = (L % 8) * (T / 8) + L / 8;
+    const int kx0 = (int)(L % ntx) * W;
+    const int ky = (int)(L / ntx);
+    const int64_t es = (int64_t)A.P * N;                       // z stride
+    const int64_t base = (int64_t)ky * A.P + kx0;
+    const int w = threadIdx.x % W;
+    const int kx = kx0 + w;
+    const bool ok = kx < A.H;
+
+    for (int t = threadIdx.x; t < N; t += NT) tw[t] = A.tw[t];
+
+    double2 S[NIN][PTS];
+#pragma unroll
+    for (int c = 0; c < NIN; c++) {
+        double2 v[PTS];
+#pragma unroll
+        for (int j = 0; j < PTS; j++) {
+            const int idx = threadIdx.x + j * NT;
+            v[j] = c2(0., 0.);
+            if (ok && (EXACT || idx < N * W)) v[j] = A.in[c * A.nkv + base + (int64_t)(idx / W) * es + w];
+        }
+#pragma unroll
+        for (int j = 0; j < PTS; j++) {
+            const int idx = threadIdx.x + j * NT;
+            if (EXACT || idx < N * W) buf[idx] = v[j];
+        }
+        __syncthreads();
+        stages<double2, N, W, true, 1, NT>(buf, tw);
+        // each thread reads back only the points it wrote, so the next
+        // component may overwrite them without another barrier
+#pragma unroll
+        for (int j = 0; j < PTS; j++) {
+            const int idx = threadIdx.x + j * NT;
+            S[c][j] = (EXACT || idx < N * W) ? buf[idx] : c2(0., 0.);
+        }
+    }
+
+    // the solve, per k point (kx, ky, kz = element index)
+    double2 O[NOUT][PTS];
+    const double gx = A.g[ok ? kx : 0];
+    const double2 kxs = A.ks[ok ? kx : 0];
+    const double gy = A.g[ky];
+    const double2 kys = A.ks[ky];
+#pragma unroll
+    for (int j = 0; j < PTS; j++) {
+        const int idx = threadIdx.x + j * NT;
+        const int kz = idx / W;
+        const bool live = ok && (EXACT || idx < N * W);
+        const double gz = A.g[live ? kz : 0];
+        const double2 kzs = A.ks[live ? kz : 0];
+        const bool zero = kx == 0 && ky == 0 && kz == 0;
+        const int64_t at = base + (int64_t)kz * es + w;
+        if constexpr (KIND == ZS_POISSON) {
+            O[0][j] = poisson_point(S[0][j], zero, zero ? 0. : gx + gy + gz, A.c0, A.c1);
+        } else if constexpr (KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR) {
+            double2 Sp[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) Sp[c] = S[c][j];
+            O[0][j] = zero ? c2(0., 0.) : project_scalar(Sp, gx, gy, gz, kxs, kys, kzs, N);
+            if constexpr (KIND == ZS_TENSOR) {
+                double2 B[3] = {c2(0., 0.), c2(0., 0.), c2(0., 0.)};
+                if (live) {
+                    if (!zero) {
+                        B[0] = A.bift[at];
+                        B[1] = A.bift[A.nkv + at];
+                        B[2] = A.bift[2 * A.nkv + at];
+                        evolve_vector(Sp, B, gx, gy, gz, kxs, kys, kzs, A.c0);
+                    }
+                    A.bift[at] = B[0];
+                    A.bift[A.nkv + at] = B[1];
+                    A.bift[2 * A.nkv + at] = B[2];
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++) O[1 + c][j] = B[c];
+            }
+        } else {
+            double2 Sv[3] = {S[0][j], S[1][j], S[2][j]};
+            if (zero) {
+                Sv[0] = Sv[1] = Sv[2] = c2(0., 0.);
+            } else {
+                project_vector_point(Sv, kxs, kys, kzs, gx + gy + gz, A.c0, A.c1);
+            }
+            if (live && A.bift) {
+                A.bift[at] = Sv[0];
+                A.bift[A.nkv + at] = Sv[1];
+                A.bift[2 * A.nkv + at] = Sv[2];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) O[c][j] = Sv[c];
+        }
+        if (live && A.kstore) A.kstore[at] = O[0][j];
+        if (zero && A.k0) A.k0[0] = O[0][j];
+    }
+
+    // inverse z transform of every output (conjugate, forward stages, conjugate)
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) {
+#pragma unroll
+        for (int j = 0; j < PTS; j++) {
+            const int idx = threadIdx.x + j * NT;
+            if (EXACT || idx < N * W) buf[idx] = cconj(O[o][j]);
+        }
+        __syncthreads();
+        stages<double2, N, W, true, 1, NT>(buf, tw);
+#pragma unroll
+        for (int j = 0; j < PTS; j++) {
+            const int idx = threadIdx.x + j * NT;
+            if (ok && (EXACT || idx < N * W))
+                A.out[o * A.nkv + base + (int64_t)(idx / W) * es + w] = cconj(buf[idx]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- first pass
+
+// r2c along x with prepareFTsource applied while the rows are staged: the
+// deposited source (1 or 6 components) and phi (and chi) are read, the
+// prepared values never reach HBM.  Same half-length transform and
+// post-processing as k_r2c_x.
+struct XArgs {
+    const double *phi, *chi;  // chi: scalar form only
+    const double *src;        // NC deposited components, N^3 each
+    double2 *out;             // NC k fields after the x pass
+    const double2 *tw;
+    int P;
+    int64_t nrows, n3v, nkv;
+    double c0, c1, c2, c3;  // tensor: coeff; scalar: bg, coeff, coeff2, coeff3
+};
+
+template <int N, int NC>
+__global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
+    constexpr int TPB = 256;
+    constexpr int M = N / 2;
+    constexpr int LINES = x_lines<N>();
+    constexpr int PER = (LINES * M + TPB - 1) / TPB;
+    constexpr bool EXACT = PER * TPB == LINES * M;
+    __shared__ double2 buf[LINES * M];
+    __shared__ double2 tw[M];
+    const int64_t row0 = (int64_t)blockIdx.x * LINES;
+    for (int t = threadIdx.x; t < M; t += TPB) tw[t] = A.tw[2 * t];
+    double2 v[NC][PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int idx = threadIdx.x + k * TPB;
+        const int line = idx / M, n = idx % M;
+        const int64_t row = row0 + line;
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c][k] = c2(0., 0.);
+        if (!((EXACT || idx < LINES * M) && row < A.nrows)) continue;
+        const int y = (int)(row % N), z = (int)(row / N);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int x = 2 * n + h;
+            const int64_t i = row * N + x;
+            const Nb c = nb_of(x, y, z, N);
+            if constexpr (NC == 6) {
+                double T[6], S[6];
+#pragma unroll
+                for (int q = 0; q < 6; q++) T[q] = A.src[q * A.n3v + i];
+                prep_tensor(A.phi, T, c, i, N, A.c0, S);
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    if (h == 0) v[q][k].x = S[q];
+                    else v[q][k].y = S[q];
+                }
+            } else {
+                const double r = prep_scalar(A.phi, A.chi[i], A.src[i], c, i, N, A.c0, A.c1, A.c2, A.c3);
+                if (h == 0) v[0][k].x = r;
+                else v[0][k].y = r;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int idx = threadIdx.x + k * TPB;
+            if (EXACT || idx < LINES * M) buf[idx] = v[q][k];
+        }
+        __syncthreads();
+        stages<double2, M, LINES, false, 1>(buf, tw);
+        for (int idx = threadIdx.x; idx < LINES * (M + 1); idx += TPB) {
+            const int line = idx / (M + 1), k = idx % (M + 1);
+            const int64_t row = row0 + line;
+            if (row >= A.nrows) continue;
+            const double2 Ak = buf[line * M + (k == M ? 0 : k)];
+            const double2 Bk = cconj(buf[line * M + (k == 0 ? 0 : M - k)]);
+            const double2 E = c2((Ak.x + Bk.x) * 0.5, (Ak.y + Bk.y) * 0.5);
+            const double2 D = c2((Ak.x - Bk.x) * 0.5, (Ak.y - Bk.y) * 0.5);
+            const double2 O = mul_mi(D);
+            A.out[q * A.nkv + row * A.P + k] = cadd(E, cmul(A.tw[k], O));
+        }
+        __syncthreads();  // the next component overwrites buf
+    }
+}
+
+template <int N, int NC> int launch_x(gev_ctx *c, const XArgs &A) {
+    constexpr int LINES = x_lines<N>();
+    hipLaunchKernelGGL((k_r2c_x_prep<N, NC>), dim3(gev::grid_for(A.nrows, LINES)), dim3(256), 0, c->stream, A);
+    return gev::check_hip(c, hipGetLastError(), "fused x pass launch");
+}
+
+template <int NC> int launch_x_nc(gev_ctx *c, XArgs A) {
+    A.tw = c->twiddle;
+    A.P = c->P;
+    A.nrows = (int64_t)c->N * c->N;
+    A.n3v = gev::n3(c);
+    A.nkv = gev::nk(c);
+    switch (c->N) {
+#define CASE(n) \
+    case n: return launch_x<n, NC>(c, A);
+        GEV_FFT_SIZES(CASE)
+#undef CASE
+    default:
+        return gev::fail(c, GEV_EINVAL, "unsupported FFT size");
+    }
+}
+
+// ---------------------------------------------------------------- last pass
+
+template <int N, int KIND> int launch_n(gev_ctx *c, const ZArgs &A) {
+    constexpr int W = zs_cols<N>();
+    const int H = N / 2 + 1;
+    hipLaunchKernelGGL((k_zsolve<N, KIND>), dim3((H + W - 1) / W, N), dim3(zs_threads<N>()), 0, c->stream, A);
+    return gev::check_hip(c, hipGetLastError(), "fused z solve launch");
+}
+
+template <int KIND> int launch_kind(gev_ctx *c, ZArgs A) {
+    A.g = c->gridk2;
+    A.ks = c->kshift;
+    A.tw = c->twiddle;
+    A.nkv = gev::nk(c);
+    A.P = c->P;
+    A.H = c->N / 2 + 1;
+    switch (c->N) {
+#define CASE(n) \
+    case n: return launch_n<n, KIND>(c, A);
+        GEV_FFT_SIZES(CASE)
+#undef CASE
+    default:
+        return gev::fail(c, GEV_EINVAL, "unsupported FFT size");
+    }
+}
+
+}  // namespace
+
+namespace gev {
+
+int launch_zsolve_poisson(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
+                          double2 *kstore, double2 *k0) {
+    gev::ScopedTimer t(c, "fft_z_poisson_z");
+    ZArgs A{};
+    A.in = in;
+    A.out = out;
+    A.kstore = kstore;
+    A.k0 = k0;
+    A.c0 = coeff / -((double)c->N * (double)c->N * (double)c->N);  // gevolution.hpp:511
+    A.c1 = modif;
+    return launch_kind<ZS_POISSON>(c, A);
+}
+
+int launch_zsolve_tensor(gev_ctx *c, const double2 *in, double2 *out, double a2dtau, bool evolve,
+                         double2 *kstore, double2 *k0) {
+    gev::ScopedTimer t(c, evolve ? "fft_z_tensor_z" : "fft_z_tensor_scalar_z");
+    ZArgs A{};
+    A.in = in;
+    A.out = out;
+    A.bift = c->BiFT;
+    A.kstore = kstore;
+    A.k0 = k0;
+    A.c0 = a2dtau;
+    return evolve ? launch_kind<ZS_TENSOR>(c, A) : launch_kind<ZS_TENSOR_SCALAR>(c, A);
+}
+
+int launch_x_prep_tensor(gev_ctx *c, double coeff) {
+    ScopedTimer t(c, "prepare_tensor_fft_r2c_x");
+    XArgs A{};
+    A.phi = c->phi;
+    A.src = c->Sij;
+    A.out = c->SijFT;
+    A.c0 = coeff;
+    return launch_x_nc<6>(c, A);
+}
+
+int launch_x_prep_scalar(gev_ctx *c, double bg, double coeff, double coeff2, double coeff3) {
+    ScopedTimer t(c, "prepare_scalar_fft_r2c_x");
+    XArgs A{};
+    A.phi = c->phi;
+    A.chi = c->chi;
+    A.src = c->source;
+    A.out = c->scalarFT;
+    A.c0 = bg;
+    A.c1 = coeff;
+    A.c2 = coeff2;
+    A.c3 = coeff3;
+    return launch_x_nc<1>(c, A);
+}
+
+int launch_zsolve_vector(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
+                         double2 *bift) {
+    gev::ScopedTimer t(c, "fft_z_vector_z");
+    ZArgs A{};
+    A.in = in;
+    A.out = out;
+    A.bift = bift;
+    A.c0 = coeff;
+    A.c1 = modif;
+    return launch_kind<ZS_VECTOR>(c, A);
+}
+
+}  // namespace gev

@@ -10,6 +10,8 @@
 
 #include "../../include/gev_pm.h"
 
+#define GEV_EXPORT extern "C" __attribute__((visibility("default")))
+
 namespace gev {
 
 // One complex k-space row holds kx in [0, N/2]; rows are padded to a multiple
@@ -101,8 +103,46 @@ inline unsigned grid_for(int64_t work, int block) {
 }
 
 // kernels implemented in the .hip files, called by api.hip
+// grid sizes with a compiled FFT plan (radix 2, 3, 4, 8 stages)
+#define GEV_FFT_SIZES(X) \
+    X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
 int fft_forward(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp);
 int fft_backward(gev_ctx *ctx, const double2 *kfield, double *real, int ncomp);
+// the x and y passes only (the z pass fused with a solve follows), and the
+// backward transform of a field whose z pass is already done (in place on
+// kfield, which is scratch afterwards)
+int fft_forward_xy(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp);
+int fft_backward_yx(gev_ctx *ctx, double2 *kfield, double *real, int ncomp);
+// prepareFTsource fused into the first FFT pass (fftsolve.hip): Sij -> SijFT
+// and source (+ phi, chi) -> scalarFT, after the x pass only
+int launch_x_prep_tensor(gev_ctx *c, double coeff);
+int launch_x_prep_scalar(gev_ctx *c, double bg, double coeff, double coeff2, double coeff3);
+int fft_forward_y(gev_ctx *ctx, double2 *kfield, int ncomp);  // the y pass alone, in place
+// z pass + solve + inverse z pass (fftsolve.hip); kstore / k0 optional
+int launch_zsolve_poisson(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
+                          double2 *kstore, double2 *k0);
+int launch_zsolve_tensor(gev_ctx *c, const double2 *in, double2 *out, double a2dtau, bool evolve,
+                         double2 *kstore, double2 *k0);
+int launch_zsolve_vector(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
+                         double2 *bift);
+
+// initial conditions (ic.hip)
+struct RealizeSetup {
+    uint8_t *redo;               // N*N row marks (device)
+    const float *sinc;           // N deconvolution factors (device)
+    const double *sx, *sy, *sc;  // spline knots, ordinates, half second derivatives (device)
+    int nsp;
+    uint32_t seed;
+    int ksphere, ignorekernel;
+    double coeff;
+};
+int launch_prng_fill(gev_ctx *c, uint32_t seed, uint64_t pos0, int64_t n, uint32_t *dev_out);
+int launch_realize(gev_ctx *c, const RealizeSetup &s);
+int launch_init_particles(gev_ctx *c, const float *dev_tpl, int64_t numpart, int numtile);
+int launch_ic_displace(gev_ctx *c, const double *field, double coeff);  // max -> red_u; refiles
+int launch_ic_q(gev_ctx *c, const double *field, double coeff);         // max |q|^2 -> red_u
+int launch_put_stencil(gev_ctx *c, double *field, const double v[27]);  // v[(dz+1)*9+(dy+1)*3+dx+1]
 
 int launch_poisson(gev_ctx *ctx, double coeff, double modif);
 int launch_project_scalar(gev_ctx *ctx, int add);

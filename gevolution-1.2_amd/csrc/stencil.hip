@@ -5,35 +5,14 @@
 // Expression order follows the reference literally (built with
 // -ffp-contract=off), so the results equal a CPU evaluation bit for bit.
 #include "gev_internal.h"
+#include "prep.h"
 
 namespace {
 
+using namespace gev_prep;
 constexpr int TPB = 256;
 
-struct Cell {
-    int x, y, z, xm, xp, ym, yp, zm, zp;
-};
-
-__device__ __forceinline__ Cell cell_of(int64_t i, int N) {
-    Cell c;
-    c.x = (int)(i % N);
-    int64_t r = i / N;
-    c.y = (int)(r % N);
-    c.z = (int)(r / N);
-    c.xm = c.x == 0 ? N - 1 : c.x - 1;
-    c.xp = c.x == N - 1 ? 0 : c.x + 1;
-    c.ym = c.y == 0 ? N - 1 : c.y - 1;
-    c.yp = c.y == N - 1 ? 0 : c.y + 1;
-    c.zm = c.z == 0 ? N - 1 : c.z - 1;
-    c.zp = c.z == N - 1 ? 0 : c.z + 1;
-    return c;
-}
-
-__device__ __forceinline__ int64_t L(int N, int x, int y, int z) {
-    return (int64_t)x + (int64_t)N * ((int64_t)y + (int64_t)N * z);
-}
-
-// prepareFTsource (2), gevolution.hpp:169-192, PHINONLINEAR on, ORIGINALMETRIC off
+// prepareFTsource (2), gevolution.hpp:169-192
 __global__ __launch_bounds__(TPB) void k_prepare_scalar(const double *__restrict__ phi,
                                                         const double *__restrict__ chi,
                                                         double *source, int N, int64_t n3v,
@@ -41,65 +20,25 @@ __global__ __launch_bounds__(TPB) void k_prepare_scalar(const double *__restrict
                                                         double coeff3) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= n3v) return;
-    Cell c = cell_of(i, N);
-    const double p = phi[i];
-    const double px = phi[L(N, c.xp, c.y, c.z)], mx = phi[L(N, c.xm, c.y, c.z)];
-    const double py = phi[L(N, c.x, c.yp, c.z)], my = phi[L(N, c.x, c.ym, c.z)];
-    const double pz = phi[L(N, c.x, c.y, c.zp)], mz = phi[L(N, c.x, c.y, c.zm)];
-    double r = coeff2 * (source[i] - bg);
-    r *= 1. - 2. * p;
-    r += 0.125 * (mx - px) * (mx - px);
-    r += 0.125 * (my - py) * (my - py);
-    r += 0.125 * (mz - pz) * (mz - pz);
-    r += (coeff3 - coeff) * p - coeff3 * chi[i];
-    source[i] = r;
+    const int64_t r = i / N;
+    const Nb c = nb_of((int)(i % N), (int)(r % N), (int)(r / N), N);
+    source[i] = prep_scalar(phi, chi[i], source[i], c, i, N, bg, coeff, coeff2, coeff3);
 }
 
-// prepareFTsource (1), gevolution.hpp:56-147, PHINONLINEAR on, ORIGINALMETRIC off
+// prepareFTsource (1), gevolution.hpp:56-147
 __global__ __launch_bounds__(TPB) void k_prepare_tensor(const double *__restrict__ phi,
                                                         double *Sij, int N, int64_t n3v,
                                                         double coeff) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= n3v) return;
-    Cell c = cell_of(i, N);
-    const double p = phi[i];
-    const double px = phi[L(N, c.xp, c.y, c.z)], mx = phi[L(N, c.xm, c.y, c.z)];
-    const double py = phi[L(N, c.x, c.yp, c.z)], my = phi[L(N, c.x, c.ym, c.z)];
-    const double pz = phi[L(N, c.x, c.y, c.zp)], mz = phi[L(N, c.x, c.y, c.zm)];
-    const double pxy = phi[L(N, c.xp, c.yp, c.z)];
-    const double pxz = phi[L(N, c.xp, c.y, c.zp)];
-    const double pyz = phi[L(N, c.x, c.yp, c.zp)];
-    double s;
-    s = coeff * Sij[i];
-    s += 0.5 * (px - mx) * (px - mx);
-    Sij[i] = s;
-    s = coeff * Sij[3 * n3v + i];
-    s += 0.5 * (py - my) * (py - my);
-    Sij[3 * n3v + i] = s;
-    s = coeff * Sij[5 * n3v + i];
-    s += 0.5 * (pz - mz) * (pz - mz);
-    Sij[5 * n3v + i] = s;
-    s = coeff * Sij[n3v + i];
-    s += px * py - p * pxy;
-    s += 0.5 * p * p;
-    s -= 0.5 * px * px;
-    s -= 0.5 * py * py;
-    s += 0.5 * pxy * pxy;
-    Sij[n3v + i] = s;
-    s = coeff * Sij[2 * n3v + i];
-    s += px * pz - p * pxz;
-    s += 0.5 * p * p;
-    s -= 0.5 * px * px;
-    s -= 0.5 * pz * pz;
-    s += 0.5 * pxz * pxz;
-    Sij[2 * n3v + i] = s;
-    s = coeff * Sij[4 * n3v + i];
-    s += py * pz - p * pyz;
-    s += 0.5 * p * p;
-    s -= 0.5 * py * py;
-    s -= 0.5 * pz * pz;
-    s += 0.5 * pyz * pyz;
-    Sij[4 * n3v + i] = s;
+    const int64_t r = i / N;
+    const Nb c = nb_of((int)(i % N), (int)(r % N), (int)(r / N), N);
+    double T[6], S[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) T[k] = Sij[k * n3v + i];
+    prep_tensor(phi, T, c, i, N, coeff, S);
+#pragma unroll
+    for (int k = 0; k < 6; k++) Sij[k * n3v + i] = S[k];
 }
 
 // deterministic two-level sum: fixed per-block tree, then one block over

@@ -42,7 +42,10 @@ EXPORTED = [
     "gev_project_ft_scalar", "gev_evolve_ft_vector", "gev_project_ft_scalar_evolve_vector",
     "gev_project_ft_vector", "gev_update_vel", "gev_move_particles", "gev_kick_drift",
     "gev_hconf", "gev_rungekutta4bg", "gev_next_dtau", "gev_run_cycles", "gev_timing_enable",
-    "gev_timing_reset", "gev_timing_report",
+    "gev_timing_reset", "gev_timing_report", "gev_prng_fill", "gev_load_homogeneous_template",
+    "gev_load_transfer_functions", "gev_generate_cic_kernel", "gev_generate_displacement_field",
+    "gev_initialize_particle_positions", "gev_displace_particles_ic", "gev_initialize_q_ic",
+    "gev_generate_ic_basic",
 ]
 
 
@@ -59,6 +62,12 @@ class Cosmology(ctypes.Structure):
 class SimParams(ctypes.Structure):
     _fields_ = [("gr_flag", ctypes.c_int32), ("vector_flag", ctypes.c_int32), ("Cf", _d),
                 ("steplimit", _d), ("z_relax", _d), ("fourpiG", _d), ("cosmo", Cosmology)]
+
+
+class ICParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint32), ("ksphere", ctypes.c_int32),
+                ("correct_displacement", ctypes.c_int32), ("numtile", ctypes.c_int32),
+                ("z_in", _d), ("A_s", _d), ("n_s", _d), ("k_pivot", _d), ("boxsize", _d)]
 
 
 class CycleState(ctypes.Structure):
@@ -119,6 +128,18 @@ def lib():
         L.gev_timing_enable.argtypes = [_p, _i]
         L.gev_timing_reset.argtypes = [_p]
         L.gev_timing_report.argtypes = [_p, ctypes.c_char_p, _sz]
+        L.gev_prng_fill.argtypes = [_p, ctypes.c_uint32, ctypes.c_uint64, _i64, _p]
+        L.gev_load_homogeneous_template.argtypes = [ctypes.c_char_p, _p, _i64, ctypes.POINTER(_i64)]
+        L.gev_load_transfer_functions.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _d, _d, _p, _p, _p,
+                                                  ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.gev_generate_cic_kernel.argtypes = [_p, _i, _i64, _p, ctypes.c_int32]
+        L.gev_generate_displacement_field.argtypes = [_p, _d, ctypes.c_int32, _p, _p, ctypes.c_uint32,
+                                                      ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        L.gev_initialize_particle_positions.argtypes = [_p, _i64, _p, ctypes.c_int32, _d]
+        L.gev_displace_particles_ic.argtypes = [_p, _d, _i, ctypes.POINTER(_d)]
+        L.gev_initialize_q_ic.argtypes = [_p, _d, _i, ctypes.POINTER(_d)]
+        L.gev_generate_ic_basic.argtypes = [_p, _p, _p, _i64, _p, ctypes.c_int32, _p, _p, _p, _p, _p,
+                                            ctypes.POINTER(_d), ctypes.POINTER(_d)]
         _lib = L
     return _lib
 
@@ -315,6 +336,51 @@ class Box:
                                        ctypes.byref(mv) if want_maxvel else None), "kick_drift")
         return mv.value if want_maxvel else None
 
+    # -- initial conditions (ic_basic.hpp)
+    def prng_fill(self, seed, pos0, n):
+        out = np.zeros(n, dtype=np.uint32)
+        self._chk(lib().gev_prng_fill(self._c, seed, pos0, n, _ptr(out)), "prng_fill")
+        return out
+
+    def generateCICKernel(self, field=SOURCE, pcldata=None, numtile=1):
+        if pcldata is None:
+            self._chk(lib().gev_generate_cic_kernel(self._c, field, 0, None, 1), "cic kernel")
+        else:
+            p = np.ascontiguousarray(pcldata, dtype=np.float32)
+            self._chk(lib().gev_generate_cic_kernel(self._c, field, len(p), _ptr(p), numtile), "cic kernel")
+
+    def generateDisplacementField(self, coeff, x, y, seed, ksphere, deconvolve_f=1, ignorekernel=0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        self._chk(lib().gev_generate_displacement_field(self._c, coeff, len(x), _ptr(x), _ptr(y), seed,
+                                                        ksphere, deconvolve_f, ignorekernel),
+                  "displacement field")
+
+    def initializeParticlePositions(self, pcldata, numtile, mass):
+        p = np.ascontiguousarray(pcldata, dtype=np.float32)
+        self._chk(lib().gev_initialize_particle_positions(self._c, len(p), _ptr(p), numtile, mass),
+                  "particle positions")
+
+    def displace_pcls_ic(self, coeff, field=CHI):
+        m = _d()
+        self._chk(lib().gev_displace_particles_ic(self._c, coeff, field, ctypes.byref(m)), "displace")
+        return m.value
+
+    def initialize_q_ic(self, coeff, field=CHI):
+        m = _d()
+        self._chk(lib().gev_initialize_q_ic(self._c, coeff, field, ctypes.byref(m)), "initialize q")
+        return m.value
+
+    def generateIC_basic(self, sim, ic, pcldata, k, d_tot, t_tot, d_cdm, t_cdm):
+        """Returns (maxvel, max_displacement)."""
+        p = np.ascontiguousarray(pcldata, dtype=np.float32)
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (k, d_tot, t_tot, d_cdm, t_cdm)]
+        mv, md = _d(), _d()
+        self._chk(lib().gev_generate_ic_basic(self._c, ctypes.byref(sim), ctypes.byref(ic), len(p), _ptr(p),
+                                              len(arrs[0]), *[_ptr(a) for a in arrs], ctypes.byref(mv),
+                                              ctypes.byref(md)), "generateIC_basic")
+        return mv.value, md.value
+
     def run_cycles(self, sim, state, ncycles):
         self._chk(lib().gev_run_cycles(self._c, ctypes.byref(sim), ctypes.byref(state), ncycles),
                   "run_cycles")
@@ -339,3 +405,30 @@ class Box:
 
 def next_dtau(sim, ngrid, a):
     return lib().gev_next_dtau(ctypes.byref(sim), ngrid, a)
+
+
+# ---------------------------------------------------------------- initial conditions (host side)
+
+def load_homogeneous_template(path):
+    """loadHomogeneousTemplate: float32 (n, 3) positions in units of the template box."""
+    n = _i64()
+    rc = lib().gev_load_homogeneous_template(path.encode(), None, 0, ctypes.byref(n))
+    if n.value <= 0:
+        raise GevError("cannot read template %s (%d)" % (path, rc))
+    out = np.zeros((n.value, 3), dtype=np.float32)
+    rc = lib().gev_load_homogeneous_template(path.encode(), _ptr(out), n.value, ctypes.byref(n))
+    if rc != 0:
+        raise GevError("cannot read template %s (%d)" % (path, rc))
+    return out
+
+
+def load_transfer_functions(path, qname, boxsize, h):
+    """loadTransferFunctions: (k * boxsize, d_<q>, t_<q> * boxsize / h)."""
+    cap = 1 << 16
+    k, d, t = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+    n = ctypes.c_int32()
+    rc = lib().gev_load_transfer_functions(path.encode(), qname.encode(), boxsize, h, _ptr(k), _ptr(d),
+                                           _ptr(t), cap, ctypes.byref(n))
+    if rc != 0:
+        raise GevError("cannot read transfer functions %s (%d)" % (path, rc))
+    return k[:n.value].copy(), d[:n.value].copy(), t[:n.value].copy()

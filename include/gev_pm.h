@@ -210,6 +210,80 @@ double gev_next_dtau(const gev_sim_params *p, int ngrid, double a);
    main.cpp:825-874 does.  Synchronous at the end (reads T00hom, maxvel). */
 int gev_run_cycles(gev_ctx *ctx, const gev_sim_params *p, gev_cycle_state *st, int ncycles);
 
+/* ---- basic initial conditions (ic_basic.hpp, prng_engine.hpp) ----------- */
+
+/* Outputs pos0 .. pos0+n-1 of sitmo::prng_engine after seed(seed)
+   (prng_engine.hpp:88-380; discard(z) advances the position by z), computed
+   on the device; host_out receives n values.  Synchronous. */
+int gev_prng_fill(gev_ctx *ctx, uint32_t seed, uint64_t pos0, int64_t n, uint32_t *host_out);
+
+/* loadHomogeneousTemplate (ic_basic.hpp:191-324): Gadget-2 file, type-1
+   particle positions divided by the header's BoxSize, each in [0,1].  Sets
+   *numpcl; writes 3 floats per particle to out when capacity (particles)
+   suffices, else returns GEV_EINVAL.  No context, no device work. */
+int gev_load_homogeneous_template(const char *path, float *out, int64_t capacity, int64_t *numpcl);
+
+/* loadTransferFunctions, file form (ic_basic.hpp:494-717): the columns
+   labelled k, d_<qname> and t_<qname> of a CLASS-format table; returns
+   k * boxsize, d, t * boxsize / h.  Errors as the reference's aborts (no
+   file, fewer than 2 rows, columns not found, negative or unordered k). */
+int gev_load_transfer_functions(const char *path, const char *qname, double boxsize, double h,
+                                double *k, double *tk_d, double *tk_t, int32_t capacity,
+                                int32_t *npoints);
+
+/* generateCICKernel (ic_basic.hpp:737-1052) into the real scalar field f
+   (GEV_SOURCE in the reference).  numpcl = 0 or pcldata = NULL: the standard
+   kernel; otherwise the kernel of the template tiled numtile times. */
+int gev_generate_cic_kernel(gev_ctx *ctx, gev_field f, int64_t numpcl, const float *pcldata,
+                            int32_t numtile);
+
+/* generateDisplacementField (ic_basic.hpp:1089-1379) on scalarFT in place: on
+   entry the transformed CIC kernel (unless ignorekernel), on exit the Gaussian
+   realisation whose amplitude is the natural cubic spline through (x[i], y[i])
+   (gsl_interp_cspline) at 2 pi |k|.  The Gaussian factor uses the correctly
+   rounded single-precision log of r1; the reference's logf may differ by one
+   float ulp. */
+int gev_generate_displacement_field(gev_ctx *ctx, double coeff, int32_t nknots, const double *x,
+                                    const double *y, uint32_t seed, int32_t ksphere,
+                                    int32_t deconvolve_f, int32_t ignorekernel);
+
+/* initializeParticlePositions (ic_basic.hpp:1400-1434): numpart * numtile^3
+   particles at (tile + template) / numtile with ID = i + numpart (xt +
+   numtile (yt + numtile zt)), q = 0, species mass `mass`; filed into cells. */
+int gev_initialize_particle_positions(gev_ctx *ctx, int64_t numpart, const float *partdata,
+                                      int32_t numtile, double mass);
+
+/* moveParticles(displace_pcls_ic_basic, coeff, &f, 1, NULL, &max, MAX, 1)
+   (ic_basic.hpp:60-116, 1995): pos += coeff grad(f), wrapped and re-filed;
+   *max_displacement (may be NULL; synchronous if not) = max coeff |grad f|. */
+int gev_displace_particles_ic(gev_ctx *ctx, double coeff, gev_field f, double *max_displacement);
+
+/* updateVel(initialize_q_ic_basic, coeff, &f, 1) (ic_basic.hpp:118-173,
+   2065): q = -coeff grad(f); *maxvel (may be NULL) = sqrt(max |q|^2). */
+int gev_initialize_q_ic(gev_ctx *ctx, double coeff, gev_field f, double *maxvel);
+
+typedef struct {
+    uint32_t seed;
+    int32_t ksphere;              /* k-domain = sphere                  */
+    int32_t correct_displacement; /* template-aware CIC kernel          */
+    int32_t numtile;              /* tiling factor of the template      */
+    double z_in;                  /* initial redshift                   */
+    double A_s, n_s, k_pivot;     /* primordial spectrum; k_pivot 1/Mpc */
+    double boxsize;               /* Mpc/h                              */
+} gev_ic_params;
+
+/* generateIC_basic (ic_basic.hpp:1626-2259), transfer-function path for one
+   CDM species with the baryons folded in (baryon treatment "ignore") and no
+   non-CDM species.  k, d_tot, t_tot, d_cdm, t_cdm: ntk rows as returned by
+   gev_load_transfer_functions for "tot" and "cdm".  Leaves the particles
+   (filed), phi, chi, Bi and BiFT as the reference leaves them; *maxvel as
+   main.cpp:326-330 turns it (GR: v / sqrt(v^2 + 1)); *max_displacement in
+   box units.  Either pointer may be NULL. */
+int gev_generate_ic_basic(gev_ctx *ctx, const gev_sim_params *p, const gev_ic_params *ic,
+                          int64_t numpcl, const float *pcldata, int32_t ntk, const double *k,
+                          const double *d_tot, const double *t_tot, const double *d_cdm,
+                          const double *t_cdm, double *maxvel, double *max_displacement);
+
 /* ---- timing (the -DBENCHMARK categories of main.cpp:71-88) --------------- */
 
 /* When enabled, every kernel launch is bracketed by HIP events on the context

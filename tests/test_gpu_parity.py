@@ -334,9 +334,95 @@ def run_both(gev, orc, N, boxsize, ncycles, gr=True, elliptic=False, seed=31, np
         assert np.array_equal(cells[order], ob.cell)
         assert abs(sg.T00hom - so.T00hom) <= 1e-12 * abs(so.T00hom) or not gr
         assert abs(sg.maxvel - so.maxvel) <= 1e-10 * abs(so.maxvel)
+        assert abs(sg.phi_k0 - so.phi_k0) <= 1e-10 * abs(so.phi_k0) + 1e-300
+        assert rel(b.download_field(gev.BIFT), ob.BiFT) < 1e-10, (c, "BiFT")
     b.close()
     ob.close()
     return out
+
+
+def cycle_by_calls(gev, b, sp, sim_g, st):
+    """One cycle through the unfused entry points, in main.cpp's order and
+    with its coefficients, host scalars advanced as gev_run_cycles does."""
+    N = b.N
+    dx = 1.0 / N
+    fpG = sp["fourpiG"]
+    cs = sim_g.cosmo
+    a = st["a"]
+    gr = sp["gr_flag"] > 0
+    b.projection_init(gev.SOURCE)
+    b.projection_init(gev.SIJ)
+    if gr:
+        b.projection_T00_project(a, True, 1.0)
+    else:
+        b.scalarProjectionCIC_project(1.0)
+    b.projection_Tij_project(a, True, 1.0)
+    if sp["vector_flag"] == 1:
+        b.projection_init(gev.BI)
+        b.projection_T0i_project(True, 1.0)
+    if gr:
+        if st["dtau_old"] > 0.:
+            H = gev.hconf(a, fpG, cs)
+            b.prepareFTsource_scalar(cs.Omega_cdm + cs.Omega_b, 3. * H * dx * dx / st["dtau_old"],
+                                     fpG * dx * dx / a, 3. * H * H * dx * dx)
+            b.fft(gev.PLAN_SOURCE, gev.FFT_FORWARD)
+            b.solveModifiedPoissonFT(1. / (dx * dx), 3. * gev.hconf(a, fpG, cs) / st["dtau_old"])
+            b.fft(gev.PLAN_PHI, gev.FFT_BACKWARD)
+    else:
+        b.fft(gev.PLAN_SOURCE, gev.FFT_FORWARD)
+        b.solveModifiedPoissonFT(fpG / a, 0.)
+        b.fft(gev.PLAN_PHI, gev.FFT_BACKWARD)
+    b.prepareFTsource_tensor(2. * fpG * dx * dx / a)
+    b.fft(gev.PLAN_SIJ, gev.FFT_FORWARD)
+    b.projectFTscalar(0)
+    b.fft(gev.PLAN_CHI, gev.FFT_BACKWARD)
+    if sp["vector_flag"] == 1:
+        b.fft(gev.PLAN_BI, gev.FFT_FORWARD)
+        b.projectFTvector(fpG * dx * dx, 0.)
+    else:
+        b.evolveFTvector(a * a * st["dtau_old"])
+    if gr:
+        b.fft(gev.PLAN_BI, gev.FFT_BACKWARD)
+    nfk = (3 if 1. / a < sp["z_relax"] + 1. else 2) if gr else 1
+    b.updateVel(gev.Q_GR if gr else gev.Q_NEWTON, (st["dtau"] + st["dtau_old"]) / 2., nfk,
+                [a, a * a * N], want_maxvel=False)
+    a = gev.rungekutta4bg(a, fpG, cs, 0.5 * st["dtau"])
+    nfd = (3 if 1. / a < sp["z_relax"] + 1. else 0) if gr else 0
+    b.moveParticles(gev.POS_GR if gr else gev.POS_NEWTON, st["dtau"], nfd, [a, a * a * N])
+    a = gev.rungekutta4bg(a, fpG, cs, 0.5 * st["dtau"])
+    st["tau"] += st["dtau"]
+    st["dtau_old"] = st["dtau"]
+    st["a"] = a
+    st["dtau"] = gev.next_dtau(sim_g, N, a)
+
+
+@pytest.mark.parametrize("N,gr,elliptic", [(32, True, False), (48, True, False), (32, True, True),
+                                           (32, False, False), (16, True, False)])
+def test_fused_cycle_equals_call_sequence(gev, N, gr, elliptic):
+    """gev_run_cycles runs the k-space solves inside the FFT z passes and the
+    kick and drift in one particle pass; the unfused entry points, called in
+    main.cpp's order, must give the same bits for every carried field."""
+    st = gr_state(N, 5)
+    sp = sim_params(N, 5.0 * N, gr=gr, vector_elliptic=elliptic)
+    sim_g = gev.SimParams(sp["gr_flag"], sp["vector_flag"], sp["Cf"], sp["steplimit"], sp["z_relax"],
+                          sp["fourpiG"], gev.make_cosmology())
+    a0 = 1.0 / (1.0 + sp["z_in"])
+    dt0 = gev.next_dtau(sim_g, N, a0)
+    fused, calls = new_box(gev, st, N), new_box(gev, st, N)
+    sg = gev.CycleState(a0, 0.0, dt0, 0.0, 0.0, 0.0, 0.0, 0, 0)
+    sc = {"a": a0, "tau": 0.0, "dtau": dt0, "dtau_old": 0.0}
+    for cyc in range(3):
+        fused.run_cycles(sim_g, sg, 1)
+        cycle_by_calls(gev, calls, sp, sim_g, sc)
+        assert sg.a == sc["a"] and sg.dtau == sc["dtau"]
+        for f in (gev.PHI, gev.CHI, gev.BI, gev.BIFT):
+            x, y = fused.download_field(f), calls.download_field(f)
+            assert np.array_equal(x, y), (cyc, f, rel(x, y))
+        pf, qf, idf = fused.download_particles()
+        pc, qc, idc = calls.download_particles()
+        assert np.array_equal(pf, pc) and np.array_equal(qf, qc) and np.array_equal(idf, idc), cyc
+    fused.close()
+    calls.close()
 
 
 def test_cycles_gr_parabolic_32(gev, orc):
