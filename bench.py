@@ -7,11 +7,14 @@ parabolic, FP64, z_in = 100.  A "step" is one cycle of gevolution's main
 loop (main.cpp:372-879): T00+Tij deposit, both prepareFTsource stencils, the
 12 FFTs with the k-space solves, and the fused kick + drift + re-filing.
 
-Initial state: particles on the cell-centred lattice (the reference's sc1
-crystal template tiled N/4 times) with a uniform random jitter of +-0.05
-cells and small random momenta; phi = chi = B = 0.  This is synthetic data
-of the configuration's shape; per-cycle cost does not depend on the field
-values at these redshifts.
+Initial state: the basic IC generator on the device (generateIC_basic of
+ic_basic.hpp) from a BBKS transfer table in CLASS format (bbks.py; h = 0.67,
+Omega_m = 0.31, A_s = 2.1e-9, n_s = 0.965, k_pivot = 0.05/Mpc), the
+reference's sc1 crystal template tiled N/4 times, Gaussian modes from seed 7,
+k-domain sphere, template-corrected displacement.  The IC generation itself
+is timed once and reported beside the cycle throughput ("ic_generator").
+`--state lattice` instead starts from the bare cell-centred lattice with a
++-0.05-cell jitter and small random momenta (the CPU baseline's sample).
 
 Output: one JSON line.  `value` is particle-steps/s with everything resident
 in HBM; `roofline` is the kernel with the largest share of the timed region,
@@ -58,6 +61,13 @@ def algorithmic_bytes(name, N, Np):
         "fft_c2c_z_inv": 32 * nk,
         "solve_modified_poisson_ft": 32 * nk,
         "project_scalar_evolve_vector": 13 * 16 * nk,       # SijFT(6)+BiFT(3) read, chiFT+BiFT(3) written
+        # fused passes (fftsolve.hip)
+        "prepare_tensor_fft_r2c_x": 7 * 8 * n3 + 6 * 16 * nk,   # phi + Tij(6) read, 6 k rows written
+        "prepare_scalar_fft_r2c_x": 3 * 8 * n3 + 16 * nk,       # phi, chi, source read, 1 k row written
+        "fft_z_poisson_z": 2 * 16 * nk,                         # y-passed source in, z-inverted phi out
+        "fft_z_tensor_z": 16 * 16 * nk,                         # Sij(6)+BiFT(3) in, BiFT(3)+chi+B(3) out
+        "fft_z_tensor_scalar_z": 7 * 16 * nk,
+        "fft_z_vector_z": 9 * 16 * nk,
         "field_sum": 8 * n3,
         "filing_scan": 8 * n3,
         "filing_scatter": 12 * Np,
@@ -66,22 +76,22 @@ def algorithmic_bytes(name, N, Np):
     return table.get(name)
 
 
-# algorithmic bytes of one whole GR-parabolic cycle at the fusion boundaries of
-# this implementation (FP64 fields, FP64 particles)
+# Minimum bytes of one GR-parabolic cycle (FP64 fields and particles): every
+# carried field and particle array read once and written once per stage that
+# has to materialise it; the FFT chains count as one read of their inputs and
+# one write of their outputs (no pass intermediates, no k-space copies except
+# the carried BiFT).
 def cycle_bytes(N, Np):
     n3 = N ** 3
     nk = N * N * (N // 2 + 1)
-    fft_f = 8 * n3 + 16 * nk
-    return (48 * Np + 64 * n3          # deposit T00 + Tij
-            + 32 * n3                  # prepare scalar (+ T00hom reads source: 8 more)
-            + 8 * n3                   # T00hom
-            + fft_f + 32 * nk + fft_f  # source r2c, Poisson, phi c2r
-            + 104 * n3                 # prepare tensor
-            + 6 * fft_f                # Sij r2c
-            + 13 * 16 * nk             # projectFTscalar + evolveFTvector
-            + fft_f + 3 * fft_f        # chi c2r, Bi c2r
-            + 96 * Np + 40 * n3        # kick + drift
-            + 8 * n3)                  # cell key + counts
+    return (48 * Np + 8 * n3 + 56 * n3  # deposit: pos, q, phi in; T00 + Tij out
+            + 8 * n3                    # T00hom: source re-read
+            + 24 * n3 + 8 * n3          # scalar chain: phi, chi, source in; phi out
+            + 56 * n3 + 48 * nk         # tensor chain: phi, Tij in; BiFT in
+            + 48 * nk + 32 * n3         # ... BiFT, chi, B out
+            + 96 * Np + 40 * n3         # kick + drift: pos, q in and out; phi, chi, B in
+            + 8 * Np + 4 * n3           # cell + slot out; counts
+            + 8 * n3 + 12 * Np)         # CSR scan; perm scatter
 
 
 def measured_traffic(path, kernel):
@@ -109,6 +119,24 @@ def synthetic_state(N, seed):
     q = rng.standard_normal(size=(Np, 3)) * 2e-6
     ids = np.arange(Np, dtype=np.int64)
     return pos, q, ids
+
+
+def ic_state(gev, box, N, boxsize, seed):
+    """Runs the device IC generator; returns (sim, state, ic_ms, maxvel, max_disp)."""
+    import bbks
+    h = 0.67
+    k, d, t = bbks.bbks_table(np.logspace(-5.0, np.log10(300.0), 400), 100.0, 0.31, h)
+    kb, tb = k * boxsize, t * boxsize / h  # loadTransferFunctions units
+    tpl = gev.load_homogeneous_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
+    sim, st = sim_setup(gev, N, boxsize)
+    ic = gev.ICParams(seed, 1, 1, N // 4, 100.0, 2.1e-9, 0.965, 0.05, boxsize)
+    box.synchronize()
+    t0 = time.perf_counter()
+    mv, md = box.generateIC_basic(sim, ic, tpl, kb, d, tb, d, tb)
+    box.synchronize()
+    ic_ms = 1e3 * (time.perf_counter() - t0)
+    st.maxvel = mv
+    return sim, st, ic_ms, mv, md
 
 
 def sim_setup(gev, N, boxsize):
@@ -163,6 +191,9 @@ def main():
     ap.add_argument("--no-timers", action="store_true", help="time without per-kernel events")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_gr_cycle_512_fp64.txt"),
                     help="measured HBM bytes per launch (rocprofv3 --pmc summary)")
+    ap.add_argument("--state", choices=["ic", "lattice"], default="ic",
+                    help="initial state: device IC generator (default) or the jittered lattice")
+    ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 1 GiB device copies (scale for the PMC byte counters)")
     args = ap.parse_args()
@@ -189,11 +220,21 @@ def main():
     Np = N ** 3
     boxsize = args.boxsize if args.boxsize else 4.0 * N
     t_setup = time.perf_counter()
-    pos, q, ids = synthetic_state(N, 7)
     box = gev.Box(N, Np)
-    box.upload_particles(pos, q, ids, 0.31 / Np)
-    del pos, q, ids
-    sim, st = sim_setup(gev, N, boxsize)
+    ic_info = None
+    if args.state == "ic":
+        sim, st, ic_ms, mv0, md0 = ic_state(gev, box, N, boxsize, args.seed)
+        ic_info = {"ms": round(ic_ms, 1), "particles": Np, "ngrid": N, "seed": args.seed,
+                   "maxvel": mv0, "max_displacement_cells": round(md0 * N, 4),
+                   "what": "generateIC_basic (ic_basic.hpp:1626-2259): 3 realisations, 15 FFTs, "
+                           "T0i and Tij deposits, particle lattice, displacement and momenta"}
+        print("IC generator: %.1f ms (max displacement %.3f cells, maxvel %.3e)" % (ic_ms, md0 * N, mv0),
+              file=sys.stderr, flush=True)
+    else:
+        pos, q, ids = synthetic_state(N, args.seed)
+        box.upload_particles(pos, q, ids, 0.31 / Np)
+        del pos, q, ids
+        sim, st = sim_setup(gev, N, boxsize)
     box.synchronize()
     t_setup = time.perf_counter() - t_setup
     print("setup %.1f s (N=%d, box %.0f Mpc/h)" % (t_setup, N, boxsize), file=sys.stderr, flush=True)
@@ -257,7 +298,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
+        "data": ("synthetic: device-generated Gaussian ICs (BBKS table, seed %d)" % args.seed
+                 if args.state == "ic" else "synthetic: jittered lattice"),
         "config": {"workload": "GR PM cycle (main.cpp:372-879), %d^3 grid, %d^3 CDM particles, box %.0f Mpc/h, "
                                "LCDM h=0.67 Om=0.31, Cf 48, steplimit 0.04, parabolic B, FP64 plain form"
                                % (N, N, boxsize),
@@ -269,6 +311,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": breakdown,
+        "ic_generator": ic_info,
     }
     box.close()
     print(json.dumps(out))
