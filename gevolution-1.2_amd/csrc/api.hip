@@ -108,7 +108,7 @@ static int64_t footprint(int N, int64_t cap) {
     int64_t b = 0;
     b += 12 * s.n3 * 8;             // phi, chi, source, Sij(6), Bi(3)
     b += 11 * s.nk * 16;            // scalarFT, SijFT(6), BiFT(3), fft scratch
-    b += s.cap * (6 * 8 + 8 + 3 * 4);  // pos, q, id, cell, slot, perm
+    b += s.cap * (2 * (6 * 8 + 8) + 3 * 4);  // pos, q, id (two sets), cell, slot, perm
     b += s.n3 * 4 + (s.n3 + 1) * 4 + s.nblk_scan * 4;
     b += s.nred * 8 + 8;
     b += N * (16 + 8 + 16);
@@ -117,7 +117,7 @@ static int64_t footprint(int N, int64_t cap) {
 
 static void free_all(gev_ctx *c) {
     void *ptrs[] = {c->phi, c->chi, c->source, c->Sij, c->Bi, c->scalarFT, c->SijFT, c->BiFT,
-                    c->fft_tmp, c->twiddle, c->gridk2, c->kshift, c->pos, c->q, c->id, c->cell,
+                    c->fft_tmp, c->twiddle, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
                     c->slot, c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -199,6 +199,9 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     A((void **)&c->pos, 3 * s.cap * 8);
     A((void **)&c->q, 3 * s.cap * 8);
     A((void **)&c->id, s.cap * 8);
+    A((void **)&c->pos2, 3 * s.cap * 8);
+    A((void **)&c->q2, 3 * s.cap * 8);
+    A((void **)&c->id2, s.cap * 8);
     A((void **)&c->cell, s.cap * 4);
     A((void **)&c->slot, s.cap * 4);
     A((void **)&c->perm, s.cap * 4);

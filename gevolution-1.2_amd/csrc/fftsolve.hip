@@ -91,21 +91,44 @@ __global__ __launch_bounds__(zs_threads<N>()) void k_zsolve(ZArgs A) {
 
     for (int t = threadIdx.x; t < N; t += NT) tw[t] = A.tw[t];
 
-    double2 S[NIN][PTS];
-#pragma unroll
-    for (int c = 0; c < NIN; c++) {
-        double2 v[PTS];
+    // column loads of input component c (element kz = idx / W of column w)
+    auto load_comp = [&](int c, double2 *v) {
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
             v[j] = c2(0., 0.);
             if (ok && (EXACT || idx < N * W)) v[j] = A.in[c * A.nkv + base + (int64_t)(idx / W) * es + w];
         }
+    };
+    // the carried B of the tensor solve, fetched while the last input transforms
+    double2 Bold[KIND == ZS_TENSOR ? 3 : 1][PTS];
+    auto load_bold = [&]() {
+#pragma unroll
+        for (int j = 0; j < PTS; j++) {
+            const int idx = threadIdx.x + j * NT;
+            const int64_t at = base + (int64_t)(idx / W) * es + w;
+#pragma unroll
+            for (int c = 0; c < (KIND == ZS_TENSOR ? 3 : 1); c++) {
+                Bold[c][j] = c2(0., 0.);
+                if (KIND == ZS_TENSOR && ok && (EXACT || idx < N * W)) Bold[c][j] = A.bift[c * A.nkv + at];
+            }
+        }
+    };
+
+    double2 S[NIN][PTS];
+    double2 v[PTS];
+    load_comp(0, v);
+#pragma unroll
+    for (int c = 0; c < NIN; c++) {
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
             if (EXACT || idx < N * W) buf[idx] = v[j];
         }
+        // the next component's loads stay in flight through this component's
+        // transform (the stage barriers wait for LDS traffic only)
+        if (c + 1 < NIN) load_comp(c + 1, v);
+        else if (KIND == ZS_TENSOR) load_bold();
         __syncthreads();
         stages<double2, N, W, true, 1, NT>(buf, tw);
         // each thread reads back only the points it wrote, so the next
@@ -143,9 +166,9 @@ __global__ __launch_bounds__(zs_threads<N>()) void k_zsolve(ZArgs A) {
                 double2 B[3] = {c2(0., 0.), c2(0., 0.), c2(0., 0.)};
                 if (live) {
                     if (!zero) {
-                        B[0] = A.bift[at];
-                        B[1] = A.bift[A.nkv + at];
-                        B[2] = A.bift[2 * A.nkv + at];
+                        B[0] = Bold[0][j];
+                        B[1] = Bold[KIND == ZS_TENSOR ? 1 : 0][j];
+                        B[2] = Bold[KIND == ZS_TENSOR ? 2 : 0][j];
                         evolve_vector(Sp, B, gx, gy, gz, kxs, kys, kzs, A.c0);
                     }
                     A.bift[at] = B[0];

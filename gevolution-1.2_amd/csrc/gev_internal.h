@@ -50,10 +50,14 @@ struct gev_ctx {
     double *gridk2 = nullptr;   // (2N sin(pi n/N))^2
     double2 *kshift = nullptr;  // 2N sin(pi n/N) exp(-i pi n/N)
 
-    // particles, structure of arrays
+    // particles, structure of arrays.  A drifting pass writes every particle
+    // at its cell-sorted index into the second set and the sets swap, so the
+    // storage order follows the last filing.
     double *pos = nullptr;  // [3][cap]
     double *q = nullptr;    // [3][cap]
     int64_t *id = nullptr;
+    double *pos2 = nullptr, *q2 = nullptr;
+    int64_t *id2 = nullptr;
     uint32_t *cell = nullptr;   // filed cell of every particle
     uint32_t *slot = nullptr;   // rank inside the cell from the counting pass
     uint32_t *perm = nullptr;   // particle indices grouped by cell, ascending ID

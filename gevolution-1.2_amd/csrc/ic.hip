@@ -413,7 +413,7 @@ int launch_init_particles(gev_ctx *c, const float *dev_tpl, int64_t numpart, int
 }
 
 int launch_ic_displace(gev_ctx *c, const double *field, double coeff) {
-    int rc = launch_fill_zero(c, c->red_u, sizeof(unsigned long long));
+    int rc = launch_fill_zero(c, c->red_u, 16);  // the fill works in 16-byte units; red_u holds 64
     if (rc || c->n == 0) return rc;
     hipLaunchKernelGGL(k_ic_displace, dim3(grid_for(c->n, TPB)), dim3(TPB), 0, c->stream, c->pos, c->cap, c->n,
                        c->N, field, coeff, (unsigned long long *)c->red_u);
@@ -422,7 +422,7 @@ int launch_ic_displace(gev_ctx *c, const double *field, double coeff) {
 }
 
 int launch_ic_q(gev_ctx *c, const double *field, double coeff) {
-    int rc = launch_fill_zero(c, c->red_u, sizeof(unsigned long long));
+    int rc = launch_fill_zero(c, c->red_u, 16);  // the fill works in 16-byte units; red_u holds 64
     if (rc || c->n == 0) return rc;
     hipLaunchKernelGGL(k_ic_q, dim3(grid_for(c->n, TPB)), dim3(TPB), 0, c->stream, c->pos, c->q, c->cap, c->n,
                        c->N, field, coeff, (unsigned long long *)c->red_u);

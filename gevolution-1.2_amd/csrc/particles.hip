@@ -5,6 +5,8 @@
 //     listing each cell's particles in ascending ID.
 //   * updateVel / moveParticles with update_q(_Newton) / update_pos(_Newton)
 //     (gevolution.hpp:570-903), separately or fused into one particle pass.
+#include <utility>
+
 #include "gev_internal.h"
 
 namespace {
@@ -370,7 +372,8 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const double *__restrict__ phi,
     const double *__restrict__ chi, const double *__restrict__ Bi, StepArgs sa,
     unsigned long long *__restrict__ vmax, uint32_t *__restrict__ count, uint32_t *__restrict__ cell,
-    uint32_t *__restrict__ slot) {
+    uint32_t *__restrict__ slot, double *__restrict__ pos_out, double *__restrict__ q_out,
+    const int64_t *__restrict__ id_in, int64_t *__restrict__ id_out) {
     __shared__ double sf[5][FREG];
     __shared__ uint32_t s_rowbeg[KTY * KTZ], s_rowoff[KTY * KTZ];
     __shared__ unsigned long long sh[TPB / 64];
@@ -448,7 +451,8 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
                 row = r;
                 roff = rowoff[r];
             }
-        const uint32_t p = perm[s_rowbeg[row] + (j - roff)];
+        const uint32_t js = s_rowbeg[row] + (j - roff);  // cell-sorted index
+        const uint32_t p = perm[js];
         double x[3] = {pos[p], pos[cap + p], pos[2 * cap + p]};
         double qq[3] = {q[p], q[cap + p], q[2 * cap + p]};
         // filed site and ref_dist = modf(pos/dx) (Particles_gevolution.hpp:170 idiom)
@@ -471,9 +475,11 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
             else
                 r = update_q_newton<POW2>(qq, rd, fphi, fchi, sa.nfk, sa.pk, sa.dtk, dx);
             rmax = r > rmax ? r : rmax;
-            q[p] = qq[0];
-            q[cap + p] = qq[1];
-            q[2 * cap + p] = qq[2];
+            if (!DRIFT) {
+                q[p] = qq[0];
+                q[cap + p] = qq[1];
+                q[2 * cap + p] = qq[2];
+            }
         }
         if (DRIFT) {
             if (PM == 0) {
@@ -484,13 +490,18 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
             }
 #pragma unroll
             for (int l = 0; l < 3; l++) x[l] = wrap_pos<POW2>(x[l], dx, N);
-            pos[p] = x[0];
-            pos[cap + p] = x[1];
-            pos[2 * cap + p] = x[2];
+            // stored at the sorted index: the storage order becomes the cell order
+            pos_out[js] = x[0];
+            pos_out[cap + js] = x[1];
+            pos_out[2 * cap + js] = x[2];
+            q_out[js] = qq[0];
+            q_out[cap + js] = qq[1];
+            q_out[2 * cap + js] = qq[2];
+            id_out[js] = id_in[p];
             const uint32_t c = (uint32_t)L(N, cell1<POW2>(x[0], dx, N), cell1<POW2>(x[1], dx, N),
                                            cell1<POW2>(x[2], dx, N));
-            cell[p] = c;
-            slot[p] = atomicAdd(count + c, 1u);
+            cell[js] = c;
+            slot[js] = atomicAdd(count + c, 1u);
         }
     }
     if (KICK) {
@@ -518,11 +529,13 @@ void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
     if ((N & (N - 1)) == 0)
         hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, true>), dim3((unsigned)nblk), dim3(TPB), 0,
                            ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot);
+                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot,
+                           ctx->pos2, ctx->q2, ctx->id, ctx->id2);
     else
         hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, false>), dim3((unsigned)nblk), dim3(TPB), 0,
                            ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot);
+                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot,
+                           ctx->pos2, ctx->q2, ctx->id, ctx->id2);
 }
 
 }  // namespace
@@ -609,6 +622,12 @@ int launch_particle_update(gev_ctx *ctx, bool kick, int qm, double dtk, int nfk,
     }
     rc = check_hip(ctx, hipGetLastError(), "particle update launch");
     if (rc) return rc;
+    if (drift && ctx->n > 0) {
+        // the drift wrote the particles in cell order into the second set
+        std::swap(ctx->pos, ctx->pos2);
+        std::swap(ctx->q, ctx->q2);
+        std::swap(ctx->id, ctx->id2);
+    }
     if (drift) {
         rc = finish_filing(ctx);
         if (rc) return rc;

@@ -129,6 +129,7 @@ int launch_field_sum(gev_ctx *ctx, const double *f, int64_t count, double *out_h
 
 int launch_fill_zero(gev_ctx *ctx, void *ptr, size_t bytes) {
     // all buffers are multiples of 16 bytes
+    if (bytes % 16 != 0 || bytes == 0) return fail(ctx, GEV_EINVAL, "fill size must be a positive multiple of 16");
     const int64_t n2 = (int64_t)(bytes / 16);
     ScopedTimer t(ctx, "projection_init");
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n2, TPB)), dim3(TPB), 0, ctx->stream,
