@@ -39,21 +39,21 @@ void timer_begin(gev_ctx *ctx, const char *name) {
     r.beg = take_event(ctx);
     r.end = take_event(ctx);
     if (!r.beg || !r.end) return;
-    hipEventRecord(r.beg, ctx->stream);
+    (void)hipEventRecord(r.beg, ctx->stream);
     ctx->pending.push_back(r);
 }
 
 void timer_end(gev_ctx *ctx) {
     if (!ctx->timing || ctx->pending.empty()) return;
-    hipEventRecord(ctx->pending.back().end, ctx->stream);
+    (void)hipEventRecord(ctx->pending.back().end, ctx->stream);
 }
 
 static void timers_collect(gev_ctx *ctx) {
     if (ctx->pending.empty()) return;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     for (auto &r : ctx->pending) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, r.beg, r.end);
+        (void)hipEventElapsedTime(&ms, r.beg, r.end);
         TimerAcc *a = nullptr;
         for (auto &x : ctx->acc)
             if (x.name == r.name) a = &x;
@@ -120,8 +120,9 @@ static void free_all(gev_ctx *c) {
                     c->fft_tmp, c->twiddle, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
                     c->slot, c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    if (c->host_pinned) hipHostFree(c->host_pinned);
+        if (p) (void)hipFree(p);
+    if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+    if (c->rec_pinned) (void)hipHostFree(c->rec_pinned);
 }
 
 // ---------------------------------------------------------------- cycle
@@ -165,7 +166,7 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     c->N = d->ngrid;
     c->P = kpitch(c->N);
     c->cap = d->npart;
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     if (d->stream) {
         c->stream = (hipStream_t)d->stream;
     } else {
@@ -211,9 +212,10 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     A((void **)&c->red_d, s.nred * 8);
     A((void **)&c->red_u, 64);
     if (ok && hipHostMalloc((void **)&c->host_pinned, 64) != hipSuccess) ok = false;
+    if (ok && hipHostMalloc((void **)&c->rec_pinned, sizeof(double) * 2 * REC_SLOTS) != hipSuccess) ok = false;
     if (!ok) {
         free_all(c);
-        if (c->own_stream) hipStreamDestroy(c->stream);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return GEV_ENOMEM;
     }
@@ -239,7 +241,7 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "create sync");
     if (rc) {
         free_all(c);
-        if (c->own_stream) hipStreamDestroy(c->stream);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return rc;
     }
@@ -249,14 +251,14 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
 
 GEV_EXPORT int gev_ctx_destroy(gev_ctx *c) {
     if (!c) return GEV_EINVAL;
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     for (auto &r : c->pending) {
-        hipEventDestroy(r.beg);
-        hipEventDestroy(r.end);
+        (void)hipEventDestroy(r.beg);
+        (void)hipEventDestroy(r.end);
     }
-    for (auto e : c->event_pool) hipEventDestroy(e);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
     free_all(c);
-    if (c->own_stream) hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }
@@ -547,7 +549,10 @@ GEV_EXPORT double gev_next_dtau(const gev_sim_params *p, int ngrid, double a) {
 // are advanced exactly as the reference does; fields and particles never
 // leave the device.  T00hom, phi(k=0) and maxvel are left on the device and
 // read once by gev_run_cycles.
-static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st) {
+// rec: when not null, a pinned slot that receives (T00hom * N^3, phi(k=0)) of
+// this cycle, copied asynchronously at the point where main.cpp:520-537
+// records the background line
+static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, double *rec) {
     const int N = c->N;
     const double dx = 1.0 / (double)N;
     const double fourpiG = p->fourpiG;
@@ -599,6 +604,11 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st) {
     rc = check_hip(c, hipMemcpyAsync(c->red_d + 1, c->scalarFT, sizeof(double),
                                      hipMemcpyDeviceToDevice, c->stream), "phi_k0");
     if (rc) return rc;
+    if (rec) {
+        rc = check_hip(c, hipMemcpyAsync(rec, c->red_d, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                       "background record");
+        if (rc) return rc;
+    }
 
     // prepareFTsource(Sij) inside the first FFT pass, projectFTscalar -> chi
     // and evolveFTvector -> BiFT inside the z passes (main.cpp:539-593).  SijFT is scratch
@@ -653,7 +663,7 @@ GEV_EXPORT int gev_run_cycles(gev_ctx *c, const gev_sim_params *p, gev_cycle_sta
         return fail(c, GEV_EINVAL, "bad simulation flags");
     if (!(st->a > 0.) || !(st->dtau > 0.)) return fail(c, GEV_EINVAL, "state needs a > 0, dtau > 0");
     for (int i = 0; i < ncycles; i++) {
-        int rc = run_cycle(c, p, st);
+        int rc = run_cycle(c, p, st, nullptr);
         if (rc) return rc;
     }
     if (ncycles > 0) {
@@ -671,6 +681,64 @@ GEV_EXPORT int gev_run_cycles(gev_ctx *c, const gev_sim_params *p, gev_cycle_sta
         st->T00hom = p->gr_flag > 0 ? h[0] / (double)n3(c) : 0.0;
         st->phi_k0 = h[1];
     }
+    return 0;
+}
+
+GEV_EXPORT int gev_run_cycles_recorded(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, int ncycles,
+                                       gev_background_record *rec) {
+    if (!c || !p || !st || ncycles < 0 || (ncycles > 0 && !rec)) return GEV_EINVAL;
+    if (p->gr_flag < 0 || p->gr_flag > 1 || p->vector_flag < 0 || p->vector_flag > 1)
+        return fail(c, GEV_EINVAL, "bad simulation flags");
+    if (!(st->a > 0.) || !(st->dtau > 0.)) return fail(c, GEV_EINVAL, "state needs a > 0, dtau > 0");
+    const double H0 = hconf(1., p->fourpiG, &p->cosmo);
+    const double n3d = (double)n3(c);
+    int done = 0;  // records whose device values have been read back
+    for (int i = 0; i < ncycles; i++) {
+        // host values of the line are those at the cycle's start (main.cpp:533)
+        rec[i].cycle = st->cycle;
+        rec[i].reserved0 = 0;
+        rec[i].tau = st->tau;
+        rec[i].a = st->a;
+        rec[i].hconf_ratio = hconf(st->a, p->fourpiG, &p->cosmo) / H0;
+        int rc = run_cycle(c, p, st, c->rec_pinned + 2 * (i % REC_SLOTS));
+        if (rc) return rc;
+        if ((i + 1) % REC_SLOTS == 0 || i + 1 == ncycles) {
+            rc = check_hip(c, hipStreamSynchronize(c->stream), "record sync");
+            if (rc) return rc;
+            for (int j = done; j <= i; j++) {
+                const double *v = c->rec_pinned + 2 * (j % REC_SLOTS);
+                rec[j].T00hom = p->gr_flag > 0 ? v[0] / n3d : 0.0;
+                rec[j].phi_k0 = v[1];
+            }
+            done = i + 1;
+        }
+    }
+    if (ncycles > 0) {
+        unsigned long long vb;
+        int rc = check_hip(c, hipMemcpy(&vb, c->red_u, sizeof(vb), hipMemcpyDeviceToHost), "cycle vmax");
+        if (rc) return rc;
+        double mv;
+        memcpy(&mv, &vb, sizeof(mv));
+        mv = sqrt(mv);
+        if (p->gr_flag > 0) mv /= sqrt(mv * mv + 1.0);
+        st->maxvel = mv;
+        st->T00hom = rec[ncycles - 1].T00hom;
+        st->phi_k0 = rec[ncycles - 1].phi_k0;
+    }
+    return 0;
+}
+
+GEV_EXPORT int gev_write_background(const char *path, const gev_background_record *rec, int n) {
+    if (!path || n < 0 || (n > 0 && !rec)) return GEV_EINVAL;
+    FILE *f = fopen(path, "a");
+    if (!f) return GEV_EINVAL;
+    for (int i = 0; i < n; i++) {
+        if (rec[i].cycle == 0)
+            fprintf(f, "# background statistics\n# cycle   tau/boxsize    a             conformal H/H0  phi(k=0)       T00(k=0)\n");
+        fprintf(f, " %6d   %e   %e   %e   %e   %e\n", rec[i].cycle, rec[i].tau, rec[i].a, rec[i].hconf_ratio,
+                rec[i].phi_k0, rec[i].T00hom);
+    }
+    fclose(f);
     return 0;
 }
 

@@ -254,6 +254,25 @@ template <int KIND> __device__ __forceinline__ bool feeds(int dxx, int dy, int d
     return true;
 }
 
+// One source cell's contribution to this thread's target with the offset
+// (DX, DY, DZ) fixed at compile time: the interior form of the loop body of
+// k_deposit_tiled (same operations, same order; the conditional terms of
+// add_particle / add_source fold away).  c: the source cell's index in the
+// tile's source region.
+template <int KIND, int DX, int DY, int DZ, typename ST>
+__device__ __forceinline__ void interior_source(Acc &A, const ST &st, const Target &T, int c, int cb, int ce,
+                                                uint32_t base, const uint32_t *s_off, const uint32_t *s_cnt,
+                                                double mass_s) {
+    if (!feeds<KIND>(DX, DY, DZ) || c < cb || c >= ce) return;
+    const uint32_t n = s_cnt[c];
+    if (n == 0) return;
+    Partial P;
+    P.clear();
+    const uint32_t r0 = s_off[c] - base;
+    for (uint32_t i = 0; i < n; i++) add_particle<KIND>(P, st.get((int)(r0 + i)), T, DX, DY, DZ);
+    add_source<KIND>(A, P, T, DX, DY, DZ, mass_s);
+}
+
 // FULL: N is a multiple of the tile, so the tile and source-region extents
 // are compile-time constants (index arithmetic without integer division).
 template <int KIND, bool USE_PHI, bool ACCUM, bool POW2, bool FULL>
@@ -281,6 +300,9 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const int nz = FULL ? TZ : min(TZ, N - z0);
     const int nsx = nx + 1, nsy = ny + 1, nsz = nz + 1;
     const int nsrc = nsx * nsy * nsz;
+    // no wrapped coordinate in this tile or its source region below it: the
+    // ascending site order of the 8 sources is the plain one for every target
+    const bool interior = x0 > 0 && y0 > 0 && z0 > 0;
     const double dx = 1.0 / (double)N, Nd = (double)N;
     const int64_t n3v = (int64_t)N * N * N;
 
@@ -387,27 +409,6 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             }
         }
     }
-    // enumeration index of each of the 8 sources, visited in site order
-    int csrc[8], dsrc[8];
-    {
-        int k = 0;
-        for (int iz = 0; iz < 2; iz++) {
-            const int dz = z > 0 ? 1 - iz : iz;
-            for (int iy = 0; iy < 2; iy++) {
-                const int dy = y > 0 ? 1 - iy : iy;
-                for (int ix = 0; ix < 2; ix++) {
-                    const int dxx = x > 0 ? 1 - ix : ix;
-                    const int px = order_pos(lx + 1 - dxx, x0, nsx);
-                    const int py = order_pos(ly + 1 - dy, y0, nsy);
-                    const int pz = order_pos(lz + 1 - dz, z0, nsz);
-                    csrc[k] = (pz * nsy + py) * nsx + px;
-                    dsrc[k] = dxx | (dy << 1) | (dz << 2);
-                    k++;
-                }
-            }
-        }
-    }
-
     // 3. chunks of whole source cells, in site order (one chunk in the
     //    common case: the 425 source cells hold fewer than CAP particles)
     int cb = 0;
@@ -469,12 +470,30 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             }
         }
         __syncthreads();
-        if (active) {
+        if (active && interior && !direct) {
+            // tiles away from coordinate 0: the 8 sources come in the fixed
+            // order (dz, dy, dx) = 111, 110 .. 000 reversed per axis, i.e.
+            // site order, with compile-time offsets
+            const int cc = (lz * nsy + ly) * nsx + lx;  // source (dx, dy, dz) = (1, 1, 1)
+            interior_source<KIND, 1, 1, 1>(A, st, T, cc, cb, ce, base, s_off, s_cnt, mass_s);
+            interior_source<KIND, 0, 1, 1>(A, st, T, cc + 1, cb, ce, base, s_off, s_cnt, mass_s);
+            interior_source<KIND, 1, 0, 1>(A, st, T, cc + nsx, cb, ce, base, s_off, s_cnt, mass_s);
+            interior_source<KIND, 0, 0, 1>(A, st, T, cc + nsx + 1, cb, ce, base, s_off, s_cnt, mass_s);
+            interior_source<KIND, 1, 1, 0>(A, st, T, cc + nsx * nsy, cb, ce, base, s_off, s_cnt, mass_s);
+            interior_source<KIND, 0, 1, 0>(A, st, T, cc + nsx * nsy + 1, cb, ce, base, s_off, s_cnt, mass_s);
+            interior_source<KIND, 1, 0, 0>(A, st, T, cc + nsx * nsy + nsx, cb, ce, base, s_off, s_cnt, mass_s);
+            interior_source<KIND, 0, 0, 0>(A, st, T, cc + nsx * nsy + nsx + 1, cb, ce, base, s_off, s_cnt, mass_s);
+        } else if (active) {
+            // the 8 sources in ascending site order: along each axis the
+            // lower neighbour (offset 1) comes first, except at coordinate 0
+            // where it wraps to N-1 and comes last
 #pragma unroll 1
             for (int k = 0; k < 8; k++) {
-                const int c = csrc[k];
+                const int ix = k & 1, iy = (k >> 1) & 1, iz = k >> 2;
+                const int dxx = x > 0 ? 1 - ix : ix, dy = y > 0 ? 1 - iy : iy, dz = z > 0 ? 1 - iz : iz;
+                const int c = (order_pos(lz + 1 - dz, z0, nsz) * nsy + order_pos(ly + 1 - dy, y0, nsy)) * nsx +
+                              order_pos(lx + 1 - dxx, x0, nsx);
                 if (c < cb || c >= ce) continue;
-                const int dxx = dsrc[k] & 1, dy = (dsrc[k] >> 1) & 1, dz = dsrc[k] >> 2;
                 if (!feeds<KIND>(dxx, dy, dz)) continue;
                 const uint32_t n = s_cnt[c];
                 if (n == 0) continue;

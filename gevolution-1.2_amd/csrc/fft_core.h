@@ -158,9 +158,13 @@ template <int N> constexpr int x_lines() {
     return l;
 }
 
-// columns per workgroup of the strided passes: 128-byte rows, at most 64 KiB
+// columns per workgroup of the strided passes: GEV_COL_BYTES-byte rows
+// (default 128), at most 64 KiB
+#ifndef GEV_COL_BYTES
+#define GEV_COL_BYTES 128
+#endif
 template <typename T, int N> constexpr int col_tile() {
-    int w = 128 / (int)(2 * sizeof(T));
+    int w = GEV_COL_BYTES / (int)(2 * sizeof(T));
     while (w > 1 && w * N * (int)(2 * sizeof(T)) > 65536) w /= 2;
     return w;
 }

@@ -39,12 +39,23 @@ template <> struct ZSolve<ZS_TENSOR> { static constexpr int NIN = 6, NOUT = 4; }
 template <> struct ZSolve<ZS_TENSOR_SCALAR> { static constexpr int NIN = 6, NOUT = 1; };
 template <> struct ZSolve<ZS_VECTOR> { static constexpr int NIN = 3, NOUT = 3; };
 
-// columns per workgroup and workgroup size: 64-byte column rows (128 at
-// N >= 1024 would need 2x the registers), and at most 4 points per thread
-template <int N> constexpr int zs_cols() { return N >= 1024 ? 2 : 4; }
-template <int N> constexpr int zs_threads() {
-    int t = N * zs_cols<N>() / 4;
-    if (t > 512) t = 512;
+// Columns per workgroup and points per thread.  Every thread keeps NIN
+// components of its points in registers through the solve, so the tensor
+// solve (6 inputs + the carried B) takes fewer points per thread, and with
+// them more threads, to keep its register count -- and the waves per SIMD
+// that hide the column loads -- in check.  GEV_ZS_COLS / GEV_ZS_PTS_TENSOR
+// override the defaults.
+#ifndef GEV_ZS_COLS
+#define GEV_ZS_COLS 4
+#endif
+#ifndef GEV_ZS_PTS_TENSOR
+#define GEV_ZS_PTS_TENSOR 4
+#endif
+template <int N> constexpr int zs_cols() { return N >= 1024 ? 2 : GEV_ZS_COLS; }
+template <int KIND> constexpr int zs_pts() { return KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR ? GEV_ZS_PTS_TENSOR : 4; }
+template <int N, int KIND> constexpr int zs_threads() {
+    int t = N * zs_cols<N>() / zs_pts<KIND>();
+    if (t > 1024) t = 1024;
     if (t < 64) t = 64;
     return (t + 63) / 64 * 64;
 }
@@ -64,9 +75,9 @@ struct ZArgs {
 };
 
 template <int N, int KIND>
-__global__ __launch_bounds__(zs_threads<N>()) void k_zsolve(ZArgs A) {
+__global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
     constexpr int W = zs_cols<N>();
-    constexpr int NT = zs_threads<N>();
+    constexpr int NT = zs_threads<N, KIND>();
     constexpr int NIN = ZSolve<KIND>::NIN, NOUT = ZSolve<KIND>::NOUT;
     constexpr int PTS = (N * W + NT - 1) / NT;
     constexpr bool EXACT = PTS * NT == N * W;
@@ -232,11 +243,23 @@ struct XArgs {
     double c0, c1, c2, c3;  // tensor: coeff; scalar: bg, coeff, coeff2, coeff3
 };
 
+// Rows per workgroup: the NC prepared components of a thread's points are
+// held in registers until each is transformed, so the 6-component form
+// takes at most GEV_XPREP_PTS points per thread and component.
+#ifndef GEV_XPREP_PTS
+#define GEV_XPREP_PTS 4
+#endif
+template <int N, int NC> constexpr int xprep_lines() {
+    int l = x_lines<N>();
+    while (NC > 1 && l > 1 && l * (N / 2) > GEV_XPREP_PTS * 256) l /= 2;
+    return l;
+}
+
 template <int N, int NC>
 __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
     constexpr int TPB = 256;
     constexpr int M = N / 2;
-    constexpr int LINES = x_lines<N>();
+    constexpr int LINES = xprep_lines<N, NC>();
     constexpr int PER = (LINES * M + TPB - 1) / TPB;
     constexpr bool EXACT = PER * TPB == LINES * M;
     __shared__ double2 buf[LINES * M];
@@ -300,7 +323,7 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
 }
 
 template <int N, int NC> int launch_x(gev_ctx *c, const XArgs &A) {
-    constexpr int LINES = x_lines<N>();
+    constexpr int LINES = xprep_lines<N, NC>();
     hipLaunchKernelGGL((k_r2c_x_prep<N, NC>), dim3(gev::grid_for(A.nrows, LINES)), dim3(256), 0, c->stream, A);
     return gev::check_hip(c, hipGetLastError(), "fused x pass launch");
 }
@@ -326,7 +349,7 @@ template <int NC> int launch_x_nc(gev_ctx *c, XArgs A) {
 template <int N, int KIND> int launch_n(gev_ctx *c, const ZArgs &A) {
     constexpr int W = zs_cols<N>();
     const int H = N / 2 + 1;
-    hipLaunchKernelGGL((k_zsolve<N, KIND>), dim3((H + W - 1) / W, N), dim3(zs_threads<N>()), 0, c->stream, A);
+    hipLaunchKernelGGL((k_zsolve<N, KIND>), dim3((H + W - 1) / W, N), dim3(zs_threads<N, KIND>()), 0, c->stream, A);
     return gev::check_hip(c, hipGetLastError(), "fused z solve launch");
 }
 

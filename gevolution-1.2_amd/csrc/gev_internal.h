@@ -12,6 +12,9 @@
 
 #define GEV_EXPORT extern "C" __attribute__((visibility("default")))
 
+// per-cycle background records buffered in pinned memory before one synchronisation
+constexpr int REC_SLOTS = 1024;
+
 namespace gev {
 
 // One complex k-space row holds kx in [0, N/2]; rows are padded to a multiple
@@ -70,6 +73,7 @@ struct gev_ctx {
     double *red_d = nullptr;              // device double scratch (sums)
     unsigned long long *red_u = nullptr;  // device max-as-bits scratch
     double *host_pinned = nullptr;        // pinned readback slot
+    double *rec_pinned = nullptr;         // pinned per-cycle (T00hom sum, phi(k=0)) pairs, REC_SLOTS of them
 
     // state: field known to be all zero (projection_init), so deposits store
     bool zeroed[5] = {false, false, false, false, false};
@@ -146,6 +150,11 @@ int launch_realize(gev_ctx *c, const RealizeSetup &s);
 int launch_init_particles(gev_ctx *c, const float *dev_tpl, int64_t numpart, int numtile);
 int launch_ic_displace(gev_ctx *c, const double *field, double coeff);  // max -> red_u; refiles
 int launch_ic_q(gev_ctx *c, const double *field, double coeff);         // max |q|^2 -> red_u
+// binned (cross) spectrum of k fields (spectrum.hip); mode 0: components
+// (comp1, comp2), 1: symmetric tensor, 2: sum over components
+int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mode, int comp1, int comp2,
+             int numbins, int deconvolve, int ktype, double *kbin, double *power, double *kscatter,
+             double *pscatter, int32_t *occupation);
 int launch_put_stencil(gev_ctx *c, double *field, const double v[27]);  // v[(dz+1)*9+(dy+1)*3+dx+1]
 
 int launch_poisson(gev_ctx *ctx, double coeff, double modif);

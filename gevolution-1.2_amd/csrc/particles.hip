@@ -45,41 +45,68 @@ __global__ __launch_bounds__(TPB) void k_count_cells(const double *__restrict__ 
     slot[p] = atomicAdd(count + c, 1u);
 }
 
-// three-phase exclusive scan of per-cell counts into CSR offsets
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_BLOCK = TPB * SCAN_ITEMS;
+// three-phase exclusive scan of per-cell counts into CSR offsets.  A block
+// covers SCAN_BLOCK = 4096 counts as SCAN_PASSES passes of TPB x 4 counts;
+// every load and store is a coalesced 16-byte access (integer sums are exact,
+// so the grouping does not change the result).
+constexpr int SCAN_PASSES = 4;
+constexpr int SCAN_BLOCK = TPB * 4 * SCAN_PASSES;
 
+// counts i .. i+3 (i a multiple of 4; in is 16-byte aligned), zero past n
+__device__ __forceinline__ uint4 load4(const uint32_t *__restrict__ in, int64_t i, int64_t n) {
+    if (i + 3 < n) return *reinterpret_cast<const uint4 *>(in + i);
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (i < n) v.x = in[i];
+    if (i + 1 < n) v.y = in[i + 1];
+    if (i + 2 < n) v.z = in[i + 2];
+    return v;
+}
+
+// exclusive scan of one value per thread over the workgroup: shuffle scan
+// inside each wavefront, wavefront totals through LDS
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
-    __shared__ uint32_t sh[TPB];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < TPB; off <<= 1) {
-        uint32_t u = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sh[threadIdx.x] += u;
-        __syncthreads();
+    __shared__ uint32_t s_w[TPB / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
     }
-    uint32_t incl = sh[threadIdx.x];
-    *total = sh[TPB - 1];
+    if (lane == 63) s_w[wave] = incl;
     __syncthreads();
-    return incl - v;
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < TPB / 64; w++) {
+        const uint32_t t = s_w[w];
+        if (w < wave) before += t;
+        tot += t;
+    }
+    __syncthreads();  // s_w is reused by the next call
+    *total = tot;
+    return before + incl - v;
 }
 
 __global__ __launch_bounds__(TPB) void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n,
                                                      uint32_t *__restrict__ part) {
-    const int64_t base = (int64_t)blockIdx.x * SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_BLOCK;
+    uint4 v[SCAN_PASSES];
+#pragma unroll
+    for (int k = 0; k < SCAN_PASSES; k++) v[k] = load4(in, b0 + 4 * ((int64_t)k * TPB + threadIdx.x), n);
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++)
-        if (base + k < n) s += in[base + k];
-    __shared__ uint32_t sh[TPB];
-    sh[threadIdx.x] = s;
+    for (int k = 0; k < SCAN_PASSES; k++) s += v[k].x + v[k].y + v[k].z + v[k].w;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    __shared__ uint32_t sh[TPB / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int o = TPB / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; w++) t += sh[w];
+        part[blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // exclusive scan of the block partials in place (single workgroup)
@@ -102,23 +129,28 @@ __global__ __launch_bounds__(TPB) void k_scan_partials(uint32_t *__restrict__ pa
 __global__ __launch_bounds__(TPB) void k_scan_final(const uint32_t *__restrict__ in, int64_t n,
                                                     const uint32_t *__restrict__ part,
                                                     uint32_t *__restrict__ out) {
-    const int64_t base = (int64_t)blockIdx.x * SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_BLOCK;
+    uint4 v[SCAN_PASSES];
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        v[k] = base + k < n ? in[base + k] : 0u;
-        s += v[k];
-    }
-    uint32_t total;
-    uint32_t off = block_exclusive_scan(s, &total) + part[blockIdx.x];
+    for (int k = 0; k < SCAN_PASSES; k++) v[k] = load4(in, b0 + 4 * ((int64_t)k * TPB + threadIdx.x), n);
+    uint32_t carry = part[blockIdx.x];
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        if (base + k < n) out[base + k] = off;
-        off += v[k];
+    for (int k = 0; k < SCAN_PASSES; k++) {
+        const int64_t i = b0 + 4 * ((int64_t)k * TPB + threadIdx.x);
+        uint32_t total;
+        const uint32_t off = carry + block_exclusive_scan(v[k].x + v[k].y + v[k].z + v[k].w, &total);
+        const uint4 o = make_uint4(off, off + v[k].x, off + v[k].x + v[k].y, off + v[k].x + v[k].y + v[k].z);
+        if (i + 3 < n) {
+            *reinterpret_cast<uint4 *>(out + i) = o;
+        } else {
+            if (i < n) out[i] = o.x;
+            if (i + 1 < n) out[i + 1] = o.y;
+            if (i + 2 < n) out[i + 2] = o.z;
+        }
+        carry += total;
     }
     // the last block also writes out[n] = grand total
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == TPB - 1) out[n] = off;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry;
 }
 
 __global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ cell,
@@ -169,6 +201,11 @@ struct FieldView {
     }
 };
 
+// Compiler-only memory fence between the phases of the particle update: the
+// LDS field values of one phase are read again in the next rather than held
+// in registers, which bounds the live register set (more waves per SIMD).
+#define GEV_LDS_PHASE() asm volatile("" ::: "memory")
+
 // update_q, gevolution.hpp:570-678 (GRADIENT_ORDER 1); returns v^2/a^2
 template <bool POW2>
 __device__ __forceinline__ double update_q(double q[3], const double rd[3], FieldView phi,
@@ -196,6 +233,7 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], Fiel
     gradphi[1] *= (v2 + e2) / e2;
     gradphi[2] *= (v2 + e2) / e2;
 
+    GEV_LDS_PHASE();
     if (nfield >= 2) {
         gradphi[0] -= (1. - rd[1]) * (1. - rd[2]) * (chi(1, 0, 0) - chi(0, 0, 0));
         gradphi[1] -= (1. - rd[0]) * (1. - rd[2]) * (chi(0, 1, 0) - chi(0, 0, 0));
@@ -213,6 +251,7 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], Fiel
 
     e2 = sqrt(e2);
 
+    GEV_LDS_PHASE();
     if (nfield >= 3) {
         pgradB[0] = ((1. - rd[2]) * (B1(1, 0, 0) - B1(0, 0, 0)) + rd[2] * (B1(1, 0, 1) - B1(0, 0, 1))) * q[1];
         pgradB[0] += ((1. - rd[1]) * (B2(1, 0, 0) - B2(0, 0, 0)) + rd[1] * (B2(1, 1, 0) - B2(0, 1, 0))) * q[2];
@@ -221,6 +260,7 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], Fiel
         pgradB[0] += (1. - rd[1]) * rd[2] * ((rd[0] - 1.) * B0(-1, 0, 1) + (1. - 2. * rd[0]) * B0(0, 0, 1) + rd[0] * B0(1, 0, 1)) * q[0];
         pgradB[0] += rd[1] * rd[2] * ((rd[0] - 1.) * B0(-1, 1, 1) + (1. - 2. * rd[0]) * B0(0, 1, 1) + rd[0] * B0(1, 1, 1)) * q[0];
 
+        GEV_LDS_PHASE();
         pgradB[1] = ((1. - rd[0]) * (B2(0, 1, 0) - B2(0, 0, 0)) + rd[0] * (B2(1, 1, 0) - B2(1, 0, 0))) * q[2];
         pgradB[1] += ((1. - rd[2]) * (B0(0, 1, 0) - B0(0, 0, 0)) + rd[2] * (B0(0, 1, 1) - B0(0, 0, 1))) * q[0];
         pgradB[1] += (1. - rd[0]) * (1. - rd[2]) * ((rd[1] - 1.) * B1(0, -1, 0) + (1. - 2. * rd[1]) * B1(0, 0, 0) + rd[1] * B1(0, 1, 0)) * q[1];
@@ -228,6 +268,7 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], Fiel
         pgradB[1] += (1. - rd[0]) * rd[2] * ((rd[1] - 1.) * B1(0, -1, 1) + (1. - 2. * rd[1]) * B1(0, 0, 1) + rd[1] * B1(0, 1, 1)) * q[1];
         pgradB[1] += rd[0] * rd[2] * ((rd[1] - 1.) * B1(1, -1, 1) + (1. - 2. * rd[1]) * B1(1, 0, 1) + rd[1] * B1(1, 1, 1)) * q[1];
 
+        GEV_LDS_PHASE();
         pgradB[2] = ((1. - rd[1]) * (B0(0, 0, 1) - B0(0, 0, 0)) + rd[1] * (B0(0, 1, 1) - B0(0, 1, 0))) * q[0];
         pgradB[2] += ((1. - rd[0]) * (B1(0, 0, 1) - B1(0, 0, 0)) + rd[0] * (B1(1, 0, 1) - B1(1, 0, 0))) * q[1];
         pgradB[2] += (1. - rd[0]) * (1. - rd[1]) * ((rd[2] - 1.) * B2(0, 0, -1) + (1. - 2. * rd[2]) * B2(0, 0, 0) + rd[2] * B2(0, 0, 1)) * q[2];
@@ -235,6 +276,7 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], Fiel
         pgradB[2] += (1. - rd[0]) * rd[1] * ((rd[2] - 1.) * B2(0, 1, -1) + (1. - 2. * rd[2]) * B2(0, 1, 0) + rd[2] * B2(0, 1, 1)) * q[2];
         pgradB[2] += rd[0] * rd[1] * ((rd[2] - 1.) * B2(1, 1, -1) + (1. - 2. * rd[2]) * B2(1, 1, 0) + rd[2] * B2(1, 1, 1)) * q[2];
 
+        GEV_LDS_PHASE();
         gradphi[0] += pgradB[0] / params[1] / e2;
         gradphi[1] += pgradB[1] / params[1] / e2;
         gradphi[2] += pgradB[2] / params[1] / e2;
@@ -438,7 +480,7 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
     uint32_t rowoff[KTY * KTZ + 1];
     rowoff[0] = 0;
 #pragma unroll
-    for (int r = 0; r < KTY * KTZ; r++) rowoff[r + 1] = rowoff[r] + s_rowoff[r];
+    for (int r = 0; r < KTY * KTZ; r++) rowoff[r + 1] = __builtin_amdgcn_readfirstlane(rowoff[r] + s_rowoff[r]);
     const uint32_t total = rowoff[KTY * KTZ];
 
     double rmax = 0.;
@@ -481,6 +523,7 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
                 q[2 * cap + p] = qq[2];
             }
         }
+        GEV_LDS_PHASE();
         if (DRIFT) {
             if (PM == 0) {
                 update_pos(x, qq, rd, fphi, fchi, fb0, fb1, fb2, sa.nfd, sa.pd, sa.dtd);

@@ -45,7 +45,9 @@ EXPORTED = [
     "gev_timing_reset", "gev_timing_report", "gev_prng_fill", "gev_load_homogeneous_template",
     "gev_load_transfer_functions", "gev_generate_cic_kernel", "gev_generate_displacement_field",
     "gev_initialize_particle_positions", "gev_displace_particles_ic", "gev_initialize_q_ic",
-    "gev_generate_ic_basic",
+    "gev_generate_ic_basic", "gev_extract_cross_spectrum", "gev_extract_power_spectrum",
+    "gev_write_power_spectrum", "gev_hibernate", "gev_restart", "gev_run_cycles_recorded",
+    "gev_write_background",
 ]
 
 
@@ -68,6 +70,11 @@ class ICParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint32), ("ksphere", ctypes.c_int32),
                 ("correct_displacement", ctypes.c_int32), ("numtile", ctypes.c_int32),
                 ("z_in", _d), ("A_s", _d), ("n_s", _d), ("k_pivot", _d), ("boxsize", _d)]
+
+
+class BackgroundRecord(ctypes.Structure):
+    _fields_ = [("cycle", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("tau", _d), ("a", _d),
+                ("hconf_ratio", _d), ("phi_k0", _d), ("T00hom", _d)]
 
 
 class CycleState(ctypes.Structure):
@@ -140,6 +147,17 @@ def lib():
         L.gev_initialize_q_ic.argtypes = [_p, _d, _i, ctypes.POINTER(_d)]
         L.gev_generate_ic_basic.argtypes = [_p, _p, _p, _i64, _p, ctypes.c_int32, _p, _p, _p, _p, _p,
                                             ctypes.POINTER(_d), ctypes.POINTER(_d)]
+        L.gev_run_cycles_recorded.argtypes = [_p, _p, _p, _i, _p]
+        L.gev_write_background.argtypes = [ctypes.c_char_p, _p, _i]
+        L.gev_hibernate.argtypes = [_p, _p, _p, ctypes.c_char_p]
+        L.gev_restart.argtypes = [_p, _p, ctypes.c_char_p, _p]
+        _i32p = ctypes.POINTER(ctypes.c_int32)
+        L.gev_extract_cross_spectrum.argtypes = [_p, _i, _i, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.c_int32, _p, _p, _p, _p, _p]
+        L.gev_extract_power_spectrum.argtypes = [_p, _i, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 _p, _p, _p, _p, _p]
+        L.gev_write_power_spectrum.argtypes = [_p, _p, _p, _p, _p, ctypes.c_int32, _d, _d, ctypes.c_char_p,
+                                               ctypes.c_char_p, _d, _d]
         _lib = L
     return _lib
 
@@ -381,6 +399,37 @@ class Box:
                                               ctypes.byref(md)), "generateIC_basic")
         return mv.value, md.value
 
+    def run_cycles_recorded(self, sim, state, ncycles):
+        """run_cycles plus one background record per cycle (main.cpp:520-537)."""
+        rec = (BackgroundRecord * max(ncycles, 1))()
+        self._chk(lib().gev_run_cycles_recorded(self._c, ctypes.byref(sim), ctypes.byref(state), ncycles, rec),
+                  "run_cycles_recorded")
+        return list(rec)[:ncycles]
+
+    # -- hibernation / restart (hibernation.hpp, ic_read.hpp)
+    def hibernate(self, sim, state, path):
+        self._chk(lib().gev_hibernate(self._c, ctypes.byref(sim), ctypes.byref(state), path.encode()), "hibernate")
+
+    def restart(self, sim, path):
+        """Returns the CycleState to continue with."""
+        st = CycleState()
+        self._chk(lib().gev_restart(self._c, ctypes.byref(sim), path.encode(), ctypes.byref(st)), "restart")
+        return st
+
+    # -- power spectra (tools.hpp)
+    def extractCrossSpectrum(self, f1, f2, numbins, deconvolve=True, ktype=1, comp1=-1, comp2=-1):
+        """Returns (kbin, power, kscatter, pscatter, occupation)."""
+        out = [np.zeros(numbins) for _ in range(4)] + [np.zeros(numbins, dtype=np.int32)]
+        self._chk(lib().gev_extract_cross_spectrum(self._c, f1, f2, numbins, int(deconvolve), ktype, comp1, comp2,
+                                                   *[_ptr(o) for o in out]), "cross spectrum")
+        return tuple(out)
+
+    def extractPowerSpectrum(self, f, numbins, deconvolve=True, ktype=1):
+        out = [np.zeros(numbins) for _ in range(4)] + [np.zeros(numbins, dtype=np.int32)]
+        self._chk(lib().gev_extract_power_spectrum(self._c, f, numbins, int(deconvolve), ktype,
+                                                   *[_ptr(o) for o in out]), "power spectrum")
+        return tuple(out)
+
     def run_cycles(self, sim, state, ncycles):
         self._chk(lib().gev_run_cycles(self._c, ctypes.byref(sim), ctypes.byref(state), ncycles),
                   "run_cycles")
@@ -408,6 +457,24 @@ def next_dtau(sim, ngrid, a):
 
 
 # ---------------------------------------------------------------- initial conditions (host side)
+
+def writePowerSpectrum(spec, rescalek, rescalep, filename, description, a, z_target=-1.0):
+    """writePowerSpectrum with EXACT_OUTPUT_REDSHIFTS (tools.hpp:268-346)."""
+    kbin, power, kscatter, pscatter, occ = [np.ascontiguousarray(v) for v in spec]
+    occ = occ.astype(np.int32)
+    rc = lib().gev_write_power_spectrum(_ptr(kbin), _ptr(power), _ptr(kscatter), _ptr(pscatter), _ptr(occ),
+                                        len(kbin), rescalek, rescalep, filename.encode(), description.encode(),
+                                        a, z_target)
+    if rc != 0:
+        raise GevError("cannot write %s (%d)" % (filename, rc))
+
+
+def write_background(path, records):
+    """Appends background records in the reference's format (main.cpp:531-533)."""
+    arr = (BackgroundRecord * max(len(records), 1))(*records)
+    if lib().gev_write_background(path.encode(), arr, len(records)) != 0:
+        raise GevError("cannot write %s" % path)
+
 
 def load_homogeneous_template(path):
     """loadHomogeneousTemplate: float32 (n, 3) positions in units of the template box."""

@@ -99,7 +99,9 @@ int gev_ctx_synchronize(gev_ctx *ctx);
    into cells (as Particles::addParticle_global does). */
 int gev_upload_particles(gev_ctx *ctx, int64_t n, const double *pos, const double *q,
                          const int64_t *id, double mass);
-/* Copies particles back in storage order (not sorted by ID). */
+/* Copies particles back in storage order (not sorted by ID).  Every pass
+   that drifts (moveParticles, kick_drift, gev_run_cycles) stores the
+   particles in the cell order of the filing it read, so the order changes. */
 int gev_download_particles(gev_ctx *ctx, double *pos, double *q, int64_t *id);
 int64_t gev_particle_count(const gev_ctx *ctx);
 /* Per-cell particle counts (N^3 uint32) and the cell index of every particle
@@ -207,7 +209,11 @@ void gev_rungekutta4bg(double *a, double fourpiG, const gev_cosmology *c, double
 double gev_next_dtau(const gev_sim_params *p, int ngrid, double a);
 
 /* Runs `ncycles` cycles of the main loop (no outputs) and advances *st as
-   main.cpp:825-874 does.  Synchronous at the end (reads T00hom, maxvel). */
+   main.cpp:825-874 does.  Synchronous at the end (reads T00hom, maxvel).
+   The cycle applies prepareFTsource inside the first FFT pass and the k-space
+   solves inside the z passes, so afterwards source and Sij hold the raw
+   deposits, SijFT is scratch and scalarFT is only kept at k = 0; phi, chi,
+   Bi, BiFT and the particles are the carried state, as in the reference. */
 int gev_run_cycles(gev_ctx *ctx, const gev_sim_params *p, gev_cycle_state *st, int ncycles);
 
 /* ---- basic initial conditions (ic_basic.hpp, prng_engine.hpp) ----------- */
@@ -283,6 +289,65 @@ int gev_generate_ic_basic(gev_ctx *ctx, const gev_sim_params *p, const gev_ic_pa
                           int64_t numpcl, const float *pcldata, int32_t ntk, const double *k,
                           const double *d_tot, const double *t_tot, const double *d_cdm,
                           const double *t_cdm, double *maxvel, double *max_displacement);
+
+/* ---- hibernation and restart (hibernation.hpp:512-612; ic_read.hpp) ----- */
+
+/* hibernate() after a gev_run_cycles call (main.cpp:827-865): writes the
+   particles, phi and chi (GR) and B / (a^2 N) (parabolic; a Newtonian run
+   first transforms BiFT back), with a, tau, the completed cycle's dtau and
+   its index, to one binary file.  Synchronous. */
+int gev_hibernate(gev_ctx *ctx, const gev_sim_params *p, const gev_cycle_state *st, const char *path);
+/* readIC's restart branch (ic_read.hpp:58-330, 671-716): loads the file into
+   a context of the same grid, rescales B by a^2 / N^2 and rebuilds BiFT by a
+   forward FFT, and sets *st so that gev_run_cycles continues the run (cycle
+   = saved + 1, dtau_old = saved dtau, dtau by ic_read.hpp:298-301). */
+int gev_restart(gev_ctx *ctx, const gev_sim_params *p, const char *path, gev_cycle_state *st);
+
+/* ---- power spectra (tools.hpp:53-346; output.hpp:1990-2028) -------------- */
+
+/* extractCrossSpectrum (tools.hpp:53-212) of two k-space fields
+   (GEV_SCALARFT, GEV_SIJFT, GEV_BIFT).  comp1, comp2 >= 0 pick one component
+   of each; otherwise a 6-component field uses the symmetric-tensor sum
+   (off-diagonals doubled) and others the sum over components.  numbins in
+   [1, 1024]; deconvolve: divide by the CIC window; ktype 0 = grid momentum
+   2N sin(pi n/N), 1 = linear 2 pi n.  Outputs per bin: mean k, mean power
+   (weighted by k^3), the k and power scatters, and the mode count, as the
+   reference's root process leaves them.  Per-bin sums are deterministic.
+   Synchronous. */
+int gev_extract_cross_spectrum(gev_ctx *ctx, gev_field f1, gev_field f2, int32_t numbins,
+                               int32_t deconvolve, int32_t ktype, int32_t comp1, int32_t comp2,
+                               double *kbin, double *power, double *kscatter, double *pscatter,
+                               int32_t *occupation);
+/* extractPowerSpectrum (tools.hpp:236-239): the cross spectrum of f with itself. */
+int gev_extract_power_spectrum(gev_ctx *ctx, gev_field f, int32_t numbins, int32_t deconvolve,
+                               int32_t ktype, double *kbin, double *power, double *kscatter,
+                               double *pscatter, int32_t *occupation);
+/* writePowerSpectrum (tools.hpp:268-346) with EXACT_OUTPUT_REDSHIFTS: when
+   1/a < z_target + 1 and `filename` already holds the spectrum written before
+   z_target was crossed, the power is interpolated linearly in redshift to
+   z_target.  ASCII table as the reference writes it.  No context. */
+int gev_write_power_spectrum(const double *kbin, const double *power, const double *kscatter,
+                             const double *pscatter, const int32_t *occupation, int32_t numbins,
+                             double rescalek, double rescalep, const char *filename,
+                             const char *description, double a, double z_target);
+
+/* One line of the background log, main.cpp:520-537: the cycle, tau and a at
+   the cycle's start, Hconf(a)/Hconf(1), phi(k=0) after the cycle's Poisson
+   solve (scalarFT(k=0).real()) and T00hom (0 for Newtonian runs). */
+typedef struct {
+    int32_t cycle;
+    int32_t reserved0;
+    double tau, a, hconf_ratio, phi_k0, T00hom;
+} gev_background_record;
+
+/* gev_run_cycles that also returns one background record per cycle in
+   rec[ncycles] (device values copied asynchronously; one synchronisation per
+   1024 cycles). */
+int gev_run_cycles_recorded(gev_ctx *ctx, const gev_sim_params *p, gev_cycle_state *st, int ncycles,
+                            gev_background_record *rec);
+/* Appends records to a background file in the reference's format
+   (main.cpp:531-533: header before the cycle-0 line).  No context. */
+int gev_write_background(const char *path, const gev_background_record *rec, int n);
 
 /* ---- timing (the -DBENCHMARK categories of main.cpp:71-88) --------------- */
 

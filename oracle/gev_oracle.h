@@ -179,6 +179,11 @@ void orc_ic_tables(const orc_sim *sim, const orc_ic *ic, int n, const double *k,
                    double *y_disp, double *y_vel);
 /* generateIC_basic, transfer-function path, one CDM species; b allocated for
    numpcl * numtile^3 particles.  maxvel as main.cpp:326-330 leaves it. */
+/* extractCrossSpectrum (tools.hpp:53-212); symmetric: 6-component tensor sum */
+void orc_extract_cross_spectrum(int N, const double *f1, const double *f2, int ncomp, int symmetric,
+                                int numbins, int deconvolve, int ktype, int comp1, int comp2, double *kbin,
+                                double *power, double *kscatter, double *pscatter, int *occupation);
+
 int orc_generate_ic_basic(const orc_sim *sim, const orc_ic *ic, int64_t numpcl, const float *pcldata,
                           int ntk, const double *k, const double *d_tot, const double *t_tot,
                           const double *d_cdm, const double *t_cdm, orc_box *b, double *maxvel,

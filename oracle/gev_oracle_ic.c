@@ -657,3 +657,4 @@ int orc_generate_ic_basic(const orc_sim *sim, const orc_ic *ic, int64_t numpcl, 
     free(tab);
     return 0;
 }
+

@@ -83,6 +83,7 @@ def lib():
         L.orc_pk_primordial.restype = _d
         L.orc_pk_primordial.argtypes = [_d, _p]
         L.orc_ic_tables.argtypes = [_p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
+        L.orc_extract_cross_spectrum.argtypes = [_i, _p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _p, _p, _p, _p]
         L.orc_generate_ic_basic.restype = _i
         L.orc_generate_ic_basic.argtypes = [_p, _p, _i64, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
         _lib = L
@@ -389,6 +390,22 @@ def ic_tables(sim, ic, k, d_tot, t_tot, d_cdm, t_cdm):
     lib().orc_ic_tables(ctypes.byref(sim), ctypes.byref(ic), n, *[_ptr(a) for a in arrs],
                         *[_ptr(o) for o in outs])
     return outs
+
+
+def extract_cross_spectrum(f1, f2=None, numbins=64, deconvolve=1, ktype=1, comp1=-1, comp2=-1,
+                           symmetric=None):
+    """extractCrossSpectrum: f1, f2 complex k fields shaped (N, N, N//2+1) or
+    (ncomp, N, N, N//2+1).  Returns (kbin, power, kscatter, pscatter, occupation)."""
+    f1 = np.ascontiguousarray(f1, dtype=np.complex128)
+    f2 = f1 if f2 is None else np.ascontiguousarray(f2, dtype=np.complex128)
+    ncomp = 1 if f1.ndim == 3 else f1.shape[0]
+    N = f1.shape[-2]
+    if symmetric is None:
+        symmetric = ncomp == 6
+    out = [np.zeros(numbins) for _ in range(4)] + [np.zeros(numbins, dtype=np.int32)]
+    lib().orc_extract_cross_spectrum(N, _ptr(f1), _ptr(f2), ncomp, int(symmetric), numbins, deconvolve, ktype,
+                                     comp1, comp2, *[_ptr(o) for o in out])
+    return tuple(out)
 
 
 def load_template(path):
