@@ -108,7 +108,7 @@ static int64_t footprint(int N, int64_t cap) {
     int64_t b = 0;
     b += 12 * s.n3 * 8;             // phi, chi, source, Sij(6), Bi(3)
     b += 11 * s.nk * 16;            // scalarFT, SijFT(6), BiFT(3), fft scratch
-    b += s.cap * (2 * (6 * 8 + 8) + 3 * 4);  // pos, q, id (two sets), cell, slot, perm
+    b += s.cap * (2 * (6 * 8 + 8) + 2 * 4);  // pos, q, id (two sets), cell, perm
     b += s.n3 * 4 + (s.n3 + 1) * 4 + s.nblk_scan * 4;
     b += s.nred * 8 + 8;
     b += N * (16 + 8 + 16);
@@ -118,7 +118,7 @@ static int64_t footprint(int N, int64_t cap) {
 static void free_all(gev_ctx *c) {
     void *ptrs[] = {c->phi, c->chi, c->source, c->Sij, c->Bi, c->scalarFT, c->SijFT, c->BiFT,
                     c->fft_tmp, c->twiddle, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
-                    c->slot, c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u};
+                    c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
@@ -204,7 +204,6 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     A((void **)&c->q2, 3 * s.cap * 8);
     A((void **)&c->id2, s.cap * 8);
     A((void **)&c->cell, s.cap * 4);
-    A((void **)&c->slot, s.cap * 4);
     A((void **)&c->perm, s.cap * 4);
     A((void **)&c->count, s.n3 * 4);
     A((void **)&c->start, (s.n3 + 1) * 4);
@@ -330,8 +329,12 @@ GEV_EXPORT int64_t gev_particle_count(const gev_ctx *c) { return c ? c->n : -1; 
 GEV_EXPORT int gev_download_filing(gev_ctx *c, uint32_t *cell_count, uint32_t *particle_cell) {
     if (!c) return GEV_EINVAL;
     int rc = check_hip(c, hipStreamSynchronize(c->stream), "filing sync");
-    if (!rc && cell_count)
-        rc = check_hip(c, hipMemcpy(cell_count, c->count, n3(c) * 4, hipMemcpyDeviceToHost), "counts");
+    if (!rc && cell_count) {
+        // per-cell counts are the differences of the CSR offsets
+        std::vector<uint32_t> st((size_t)n3(c) + 1);
+        rc = check_hip(c, hipMemcpy(st.data(), c->start, (n3(c) + 1) * 4, hipMemcpyDeviceToHost), "counts");
+        for (int64_t i = 0; !rc && i < n3(c); i++) cell_count[i] = st[i + 1] - st[i];
+    }
     if (!rc && particle_cell && c->n > 0)
         rc = check_hip(c, hipMemcpy(particle_cell, c->cell, c->n * 4, hipMemcpyDeviceToHost), "cells");
     return rc;

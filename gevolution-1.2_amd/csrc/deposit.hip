@@ -255,12 +255,11 @@ template <int KIND> __device__ __forceinline__ bool feeds(int dxx, int dy, int d
 }
 
 // One source cell's contribution to this thread's target with the offset
-// (DX, DY, DZ) fixed at compile time: the interior form of the loop body of
-// k_deposit_tiled (same operations, same order; the conditional terms of
-// add_particle / add_source fold away).  c: the source cell's index in the
-// tile's source region.
+// (DX, DY, DZ) fixed at compile time (same operations, same order as the
+// reference's local cube; the conditional terms of add_particle / add_source
+// fold away).  c: the source cell's index in the tile's source region.
 template <int KIND, int DX, int DY, int DZ, typename ST>
-__device__ __forceinline__ void interior_source(Acc &A, const ST &st, const Target &T, int c, int cb, int ce,
+__device__ __forceinline__ void add_source_cell(Acc &A, const ST &st, const Target &T, int c, int cb, int ce,
                                                 uint32_t base, const uint32_t *s_off, const uint32_t *s_cnt,
                                                 double mass_s) {
     if (!feeds<KIND>(DX, DY, DZ) || c < cb || c >= ce) return;
@@ -269,8 +268,31 @@ __device__ __forceinline__ void interior_source(Acc &A, const ST &st, const Targ
     Partial P;
     P.clear();
     const uint32_t r0 = s_off[c] - base;
+#pragma unroll 1
     for (uint32_t i = 0; i < n; i++) add_particle<KIND>(P, st.get((int)(r0 + i)), T, DX, DY, DZ);
     add_source<KIND>(A, P, T, DX, DY, DZ, mass_s);
+}
+
+// Runs CALL(dx, dy, dz) with the offset of a source from its target,
+// d = dx | dy << 1 | dz << 2, as compile-time values.  (A switch over
+// force-inlined template calls: every accumulator stays in registers.)
+#define GEV_BY_OFFSET(d, CALL) \
+    switch (d) {               \
+    case 0: CALL(0, 0, 0); break; \
+    case 1: CALL(1, 0, 0); break; \
+    case 2: CALL(0, 1, 0); break; \
+    case 3: CALL(1, 1, 0); break; \
+    case 4: CALL(0, 0, 1); break; \
+    case 5: CALL(1, 0, 1); break; \
+    case 6: CALL(0, 1, 1); break; \
+    default: CALL(1, 1, 1); break; \
+    }
+
+// the particles 0 .. m-1 of the records, in order, onto one partial sum
+template <int KIND, int DX, int DY, int DZ, typename ST>
+__device__ __forceinline__ void add_records(Partial &P, const ST &st, const Target &T, uint32_t m) {
+#pragma unroll 1
+    for (uint32_t i = 0; i < m; i++) add_particle<KIND>(P, st.get((int)i), T, DX, DY, DZ);
 }
 
 // FULL: N is a multiple of the tile, so the tile and source-region extents
@@ -468,49 +490,77 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                     st.put((int)r, make_rec<KIND, USE_PHI, POW2>(pq[k], rx, ry, rz, dx, Nd, a, mass_t));
                 }
             }
-        }
-        __syncthreads();
-        if (active && interior && !direct) {
-            // tiles away from coordinate 0: the 8 sources come in the fixed
-            // order (dz, dy, dx) = 111, 110 .. 000 reversed per axis, i.e.
-            // site order, with compile-time offsets
-            const int cc = (lz * nsy + ly) * nsx + lx;  // source (dx, dy, dz) = (1, 1, 1)
-            interior_source<KIND, 1, 1, 1>(A, st, T, cc, cb, ce, base, s_off, s_cnt, mass_s);
-            interior_source<KIND, 0, 1, 1>(A, st, T, cc + 1, cb, ce, base, s_off, s_cnt, mass_s);
-            interior_source<KIND, 1, 0, 1>(A, st, T, cc + nsx, cb, ce, base, s_off, s_cnt, mass_s);
-            interior_source<KIND, 0, 0, 1>(A, st, T, cc + nsx + 1, cb, ce, base, s_off, s_cnt, mass_s);
-            interior_source<KIND, 1, 1, 0>(A, st, T, cc + nsx * nsy, cb, ce, base, s_off, s_cnt, mass_s);
-            interior_source<KIND, 0, 1, 0>(A, st, T, cc + nsx * nsy + 1, cb, ce, base, s_off, s_cnt, mass_s);
-            interior_source<KIND, 1, 0, 0>(A, st, T, cc + nsx * nsy + nsx, cb, ce, base, s_off, s_cnt, mass_s);
-            interior_source<KIND, 0, 0, 0>(A, st, T, cc + nsx * nsy + nsx + 1, cb, ce, base, s_off, s_cnt, mass_s);
-        } else if (active) {
-            // the 8 sources in ascending site order: along each axis the
-            // lower neighbour (offset 1) comes first, except at coordinate 0
-            // where it wraps to N-1 and comes last
+            __syncthreads();
+            if (active && interior) {
+                // tiles away from coordinate 0: the 8 sources come in the
+                // fixed order (dz, dy, dx) = 111, 110 .. 000 reversed per
+                // axis, i.e. site order, with compile-time offsets
+                const int cc = (lz * nsy + ly) * nsx + lx;  // source (dx, dy, dz) = (1, 1, 1)
+                add_source_cell<KIND, 1, 1, 1>(A, st, T, cc, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 1, 1>(A, st, T, cc + 1, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 0, 1>(A, st, T, cc + nsx, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 0, 1>(A, st, T, cc + nsx + 1, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 1, 0>(A, st, T, cc + nsx * nsy, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 1, 0>(A, st, T, cc + nsx * nsy + 1, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 0, 0>(A, st, T, cc + nsx * nsy + nsx, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 0, 0>(A, st, T, cc + nsx * nsy + nsx + 1, cb, ce, base, s_off, s_cnt,
+                                               mass_s);
+            } else if (active) {
+                // the 8 sources in ascending site order: along each axis the
+                // lower neighbour (offset 1) comes first, except at
+                // coordinate 0 where it wraps to N-1 and comes last
 #pragma unroll 1
-            for (int k = 0; k < 8; k++) {
-                const int ix = k & 1, iy = (k >> 1) & 1, iz = k >> 2;
-                const int dxx = x > 0 ? 1 - ix : ix, dy = y > 0 ? 1 - iy : iy, dz = z > 0 ? 1 - iz : iz;
-                const int c = (order_pos(lz + 1 - dz, z0, nsz) * nsy + order_pos(ly + 1 - dy, y0, nsy)) * nsx +
-                              order_pos(lx + 1 - dxx, x0, nsx);
-                if (c < cb || c >= ce) continue;
-                if (!feeds<KIND>(dxx, dy, dz)) continue;
-                const uint32_t n = s_cnt[c];
-                if (n == 0) continue;
-                Partial P;
-                P.clear();
-                if (!direct) {
-                    const uint32_t r0 = s_off[c] - base;
-                    for (uint32_t i = 0; i < n; i++) add_particle<KIND>(P, st.get((int)(r0 + i)), T, dxx, dy, dz);
-                } else {
-                    const double rx = wrapi(x - dxx, N) * dx, ry = wrapi(y - dy, N) * dx, rz = wrapi(z - dz, N) * dx;
-                    for (uint32_t i = 0; i < n; i++)
-                        add_particle<KIND>(P, make_rec<KIND, USE_PHI, POW2>(load_pq<KIND>(pos, q, cap, perm[s_gbeg[c] + i]),
-                                                                            rx, ry, rz, dx, Nd, a, mass_t),
-                                           T, dxx, dy, dz);
+                for (int k = 0; k < 8; k++) {
+                    const int ix = k & 1, iy = (k >> 1) & 1, iz = k >> 2;
+                    const int dxx = x > 0 ? 1 - ix : ix, dy = y > 0 ? 1 - iy : iy, dz = z > 0 ? 1 - iz : iz;
+                    const int c = (order_pos(lz + 1 - dz, z0, nsz) * nsy + order_pos(ly + 1 - dy, y0, nsy)) * nsx +
+                                  order_pos(lx + 1 - dxx, x0, nsx);
+#define GEV_SRC(X, Y, Z) add_source_cell<KIND, X, Y, Z>(A, st, T, c, cb, ce, base, s_off, s_cnt, mass_s)
+                    GEV_BY_OFFSET(dxx | (dy << 1) | (dz << 2), GEV_SRC)
+#undef GEV_SRC
                 }
-                add_source<KIND>(A, P, T, dxx, dy, dz, mass_s);
             }
+        } else {
+            // one source cell holds more particles than a chunk: they pass
+            // through the records in pieces of CAP (in order), and each
+            // target it feeds carries its partial sum across the pieces
+            const int c = cb;
+            const uint32_t n = s_cnt[c];
+            int d = -1;  // offset of source c from this thread's target
+            if (active) {
+#pragma unroll 1
+                for (int k = 0; k < 8; k++) {
+                    const int ix = k & 1, iy = (k >> 1) & 1, iz = k >> 2;
+                    const int dxx = x > 0 ? 1 - ix : ix, dy = y > 0 ? 1 - iy : iy, dz = z > 0 ? 1 - iz : iz;
+                    const int ck = (order_pos(lz + 1 - dz, z0, nsz) * nsy + order_pos(ly + 1 - dy, y0, nsy)) * nsx +
+                                   order_pos(lx + 1 - dxx, x0, nsx);
+                    if (ck == c && feeds<KIND>(dxx, dy, dz)) d = dxx | (dy << 1) | (dz << 2);
+                }
+            }
+            const int px = c % nsx, py = (c / nsx) % nsy, pz = c / (nsx * nsy);
+            const double rx = wrapi(x0 - 1 + order_slot(px, x0, nsx), N) * dx;
+            const double ry = wrapi(y0 - 1 + order_slot(py, y0, nsy), N) * dx;
+            const double rz = wrapi(z0 - 1 + order_slot(pz, z0, nsz), N) * dx;
+            Partial P;
+            P.clear();
+            for (uint32_t s0 = 0; s0 < n; s0 += (uint32_t)CAP) {
+                const uint32_t m = min((uint32_t)CAP, n - s0);
+                for (uint32_t r = threadIdx.x; r < m; r += TPB)
+                    st.put((int)r, make_rec<KIND, USE_PHI, POW2>(load_pq<KIND>(pos, q, cap, perm[s_gbeg[c] + s0 + r]),
+                                                                  rx, ry, rz, dx, Nd, a, mass_t));
+                __syncthreads();
+#define GEV_REC(X, Y, Z) add_records<KIND, X, Y, Z>(P, st, T, m)
+                if (d >= 0) {
+                    GEV_BY_OFFSET(d, GEV_REC)
+                }
+#undef GEV_REC
+                __syncthreads();
+            }
+#define GEV_ADD(X, Y, Z) add_source<KIND>(A, P, T, X, Y, Z, mass_s)
+            if (d >= 0) {
+                GEV_BY_OFFSET(d, GEV_ADD)
+            }
+#undef GEV_ADD
         }
         __syncthreads();
         cb = ce;

@@ -51,7 +51,11 @@ template <> struct ZSolve<ZS_VECTOR> { static constexpr int NIN = 3, NOUT = 3; }
 #ifndef GEV_ZS_PTS_TENSOR
 #define GEV_ZS_PTS_TENSOR 4
 #endif
-template <int N> constexpr int zs_cols() { return N >= 1024 ? 2 : GEV_ZS_COLS; }
+template <int N> constexpr int zs_cols() { return N >= 768 ? 2 : GEV_ZS_COLS; }
+// The tensor solve prefetches the carried B into registers during the last
+// input transform where the registers allow it; at N = 384 and above 512 the
+// prefetch would spill (compiler resource report), so B is read at the solve.
+template <int N> constexpr bool zs_prefetch_b() { return N <= 512 && N != 384; }
 template <int KIND> constexpr int zs_pts() { return KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR ? GEV_ZS_PTS_TENSOR : 4; }
 template <int N, int KIND> constexpr int zs_threads() {
     int t = N * zs_cols<N>() / zs_pts<KIND>();
@@ -139,7 +143,7 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
         // the next component's loads stay in flight through this component's
         // transform (the stage barriers wait for LDS traffic only)
         if (c + 1 < NIN) load_comp(c + 1, v);
-        else if (KIND == ZS_TENSOR) load_bold();
+        else if (KIND == ZS_TENSOR && zs_prefetch_b<N>()) load_bold();
         __syncthreads();
         stages<double2, N, W, true, 1, NT>(buf, tw);
         // each thread reads back only the points it wrote, so the next
@@ -177,9 +181,15 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
                 double2 B[3] = {c2(0., 0.), c2(0., 0.), c2(0., 0.)};
                 if (live) {
                     if (!zero) {
-                        B[0] = Bold[0][j];
-                        B[1] = Bold[KIND == ZS_TENSOR ? 1 : 0][j];
-                        B[2] = Bold[KIND == ZS_TENSOR ? 2 : 0][j];
+                        if constexpr (zs_prefetch_b<N>()) {
+                            B[0] = Bold[0][j];
+                            B[1] = Bold[KIND == ZS_TENSOR ? 1 : 0][j];
+                            B[2] = Bold[KIND == ZS_TENSOR ? 2 : 0][j];
+                        } else {
+                            B[0] = A.bift[at];
+                            B[1] = A.bift[A.nkv + at];
+                            B[2] = A.bift[2 * A.nkv + at];
+                        }
                         evolve_vector(Sp, B, gx, gy, gz, kxs, kys, kzs, A.c0);
                     }
                     A.bift[at] = B[0];

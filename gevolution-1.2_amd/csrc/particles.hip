@@ -33,8 +33,7 @@ template <bool POW2> __device__ __forceinline__ int cell1(double p, double dx, d
 
 __global__ __launch_bounds__(TPB) void k_count_cells(const double *__restrict__ pos, int64_t cap,
                                                      int64_t n, int N, uint32_t *__restrict__ count,
-                                                     uint32_t *__restrict__ cell,
-                                                     uint32_t *__restrict__ slot) {
+                                                     uint32_t *__restrict__ cell) {
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= n) return;
     const double dx = 1.0 / (double)N;
@@ -42,7 +41,7 @@ __global__ __launch_bounds__(TPB) void k_count_cells(const double *__restrict__ 
     const int cz = cell1<false>(pos[2 * cap + p], dx, N);
     const uint32_t c = (uint32_t)L(N, cx, cy, cz);
     cell[p] = c;
-    slot[p] = atomicAdd(count + c, 1u);
+    atomicAdd(count + c, 1u);
 }
 
 // three-phase exclusive scan of per-cell counts into CSR offsets.  A block
@@ -126,13 +125,15 @@ __global__ __launch_bounds__(TPB) void k_scan_partials(uint32_t *__restrict__ pa
         }
 }
 
-__global__ __launch_bounds__(TPB) void k_scan_final(const uint32_t *__restrict__ in, int64_t n,
+// writes the exclusive offsets to out (CSR start) and over the counts in
+// cnt, which the scatter then advances as per-cell cursors
+__global__ __launch_bounds__(TPB) void k_scan_final(uint32_t *__restrict__ cnt, int64_t n,
                                                     const uint32_t *__restrict__ part,
                                                     uint32_t *__restrict__ out) {
     const int64_t b0 = (int64_t)blockIdx.x * SCAN_BLOCK;
     uint4 v[SCAN_PASSES];
 #pragma unroll
-    for (int k = 0; k < SCAN_PASSES; k++) v[k] = load4(in, b0 + 4 * ((int64_t)k * TPB + threadIdx.x), n);
+    for (int k = 0; k < SCAN_PASSES; k++) v[k] = load4(cnt, b0 + 4 * ((int64_t)k * TPB + threadIdx.x), n);
     uint32_t carry = part[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < SCAN_PASSES; k++) {
@@ -142,10 +143,11 @@ __global__ __launch_bounds__(TPB) void k_scan_final(const uint32_t *__restrict__
         const uint4 o = make_uint4(off, off + v[k].x, off + v[k].x + v[k].y, off + v[k].x + v[k].y + v[k].z);
         if (i + 3 < n) {
             *reinterpret_cast<uint4 *>(out + i) = o;
+            *reinterpret_cast<uint4 *>(cnt + i) = o;
         } else {
-            if (i < n) out[i] = o.x;
-            if (i + 1 < n) out[i + 1] = o.y;
-            if (i + 2 < n) out[i + 2] = o.z;
+            if (i < n) out[i] = cnt[i] = o.x;
+            if (i + 1 < n) out[i + 1] = cnt[i + 1] = o.y;
+            if (i + 2 < n) out[i + 2] = cnt[i + 2] = o.z;
         }
         carry += total;
     }
@@ -153,13 +155,13 @@ __global__ __launch_bounds__(TPB) void k_scan_final(const uint32_t *__restrict__
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry;
 }
 
-__global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ cell,
-                                                 const uint32_t *__restrict__ slot,
-                                                 const uint32_t *__restrict__ start, int64_t n,
-                                                 uint32_t *__restrict__ perm) {
+// every particle takes the next place of its cell (the in-cell order is
+// then fixed by the ID sort)
+__global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ cell, uint32_t *__restrict__ cursor,
+                                                 int64_t n, uint32_t *__restrict__ perm) {
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= n) return;
-    perm[start[cell[p]] + slot[p]] = (uint32_t)p;
+    perm[atomicAdd(cursor + cell[p], 1u)] = (uint32_t)p;
 }
 
 // ascending ID inside every cell (insertion sort; cells hold few particles)
@@ -408,16 +410,25 @@ struct StepArgs {
 // locate, optional kick, optional drift (+ wrap + count into the new cell).
 // QM / PM select the callbacks (0 = GR, 1 = Newton).  The filing
 // (start / perm) must describe the current positions.
+// GEV_KICK_WAVES > 0 asks the compiler for that many waves per SIMD
+#ifndef GEV_KICK_WAVES
+#define GEV_KICK_WAVES 0
+#endif
+#if GEV_KICK_WAVES > 0
+#define GEV_KICK_OCC __attribute__((amdgpu_waves_per_eu(GEV_KICK_WAVES)))
+#else
+#define GEV_KICK_OCC
+#endif
+
 template <bool KICK, int QM, bool DRIFT, int PM, bool POW2>
-__global__ __launch_bounds__(TPB) void k_update_tiled(
+__global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
     double *__restrict__ pos, double *__restrict__ q, int64_t cap, int N, int ntx, int nty, int ntz,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const double *__restrict__ phi,
     const double *__restrict__ chi, const double *__restrict__ Bi, StepArgs sa,
     unsigned long long *__restrict__ vmax, uint32_t *__restrict__ count, uint32_t *__restrict__ cell,
-    uint32_t *__restrict__ slot, double *__restrict__ pos_out, double *__restrict__ q_out,
+    double *__restrict__ pos_out, double *__restrict__ q_out,
     const int64_t *__restrict__ id_in, int64_t *__restrict__ id_out) {
     __shared__ double sf[5][FREG];
-    __shared__ uint32_t s_rowbeg[KTY * KTZ], s_rowoff[KTY * KTZ];
     __shared__ unsigned long long sh[TPB / 64];
 
     const int64_t nblk = (int64_t)ntx * nty * ntz;
@@ -432,7 +443,42 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
     const bool need_phi = KICK || (DRIFT && PM == 0 && sa.nfd >= 1);
     const bool need_chi = (KICK && sa.nfk >= 2) || (DRIFT && PM == 0 && sa.nfd >= 2);
     const bool need_B = (KICK && QM == 0 && sa.nfk >= 3) || (DRIFT && PM == 0 && sa.nfd >= 3);
-    // all staging loads of this thread in flight before the first LDS write
+    // rows of the tile: cells (x0 .. x0+nx-1, y, z) are one run of the
+    // permutation.  Their extents are wave-uniform (scalar loads), so every
+    // wave finds its particles without waiting for the field staging.
+    uint32_t rowbeg[KTY * KTZ], rowoff[KTY * KTZ + 1];
+    rowoff[0] = 0;
+#pragma unroll
+    for (int r = 0; r < KTY * KTZ; r++) {
+        const int ly = r % KTY, lz = r / KTY;
+        uint32_t b = 0, e = 0;
+        if (ly < ny && lz < nz) {
+            const int64_t g0 = L(N, x0, y0 + ly, z0 + lz);
+            b = start[g0];
+            e = start[g0 + nx];
+        }
+        rowbeg[r] = b;
+        rowoff[r + 1] = rowoff[r] + (e - b);
+    }
+    const uint32_t total = rowoff[KTY * KTZ];
+    // cell-sorted index of the tile's j-th particle
+    auto sorted_index = [&](uint32_t jj) {
+        uint32_t s = rowbeg[0] + jj;
+#pragma unroll
+        for (int r = 1; r < KTY * KTZ; r++)
+            if (rowoff[r] <= jj) s = rowbeg[r] + (jj - rowoff[r]);
+        return s;
+    };
+
+    // Loads in the order they are needed: this thread's first permutation
+    // entry, the field staging (no dependence), then the first particle.
+    // Loads complete in order, so the particle loads wait only for the
+    // permutation entry and the LDS writes only for the staging.
+    uint32_t j = threadIdx.x, js = 0, p = 0;
+    if (j < total) {
+        js = sorted_index(j);
+        p = perm[js];
+    }
     constexpr int PER = (FREG + TPB - 1) / TPB;
     double v[5][PER];
 #pragma unroll
@@ -450,18 +496,18 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
             }
         }
     }
-    // rows of the tile: cells (x0 .. x0+nx-1, y, z) are one run of the permutation
-    if (threadIdx.x < KTY * KTZ) {
-        const int ly = threadIdx.x % KTY, lz = threadIdx.x / KTY;
-        uint32_t b = 0, e = 0;
-        if (ly < ny && lz < nz) {
-            const int64_t g0 = L(N, x0, y0 + ly, z0 + lz);
-            b = start[g0];
-            e = start[g0 + nx];
-        }
-        s_rowbeg[threadIdx.x] = b;
-        s_rowoff[threadIdx.x] = e - b;
-    }
+    double x[3] = {0., 0., 0.}, qq[3] = {0., 0., 0.};
+    int64_t pid = 0;
+    auto load_particle = [&]() {
+        x[0] = pos[p];
+        x[1] = pos[cap + p];
+        x[2] = pos[2 * cap + p];
+        qq[0] = q[p];
+        qq[1] = q[cap + p];
+        qq[2] = q[2 * cap + p];
+        if (DRIFT) pid = id_in[p];
+    };
+    if (j < total) load_particle();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int i = threadIdx.x + k * TPB;
@@ -476,27 +522,9 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
         }
     }
     __syncthreads();
-    // row offsets (every thread scans the 16 row counts itself)
-    uint32_t rowoff[KTY * KTZ + 1];
-    rowoff[0] = 0;
-#pragma unroll
-    for (int r = 0; r < KTY * KTZ; r++) rowoff[r + 1] = __builtin_amdgcn_readfirstlane(rowoff[r] + s_rowoff[r]);
-    const uint32_t total = rowoff[KTY * KTZ];
 
     double rmax = 0.;
-    for (uint32_t j = threadIdx.x; j < total; j += TPB) {
-        int row = 0;
-        uint32_t roff = 0;
-#pragma unroll
-        for (int r = 1; r < KTY * KTZ; r++)
-            if (rowoff[r] <= j) {
-                row = r;
-                roff = rowoff[r];
-            }
-        const uint32_t js = s_rowbeg[row] + (j - roff);  // cell-sorted index
-        const uint32_t p = perm[js];
-        double x[3] = {pos[p], pos[cap + p], pos[2 * cap + p]};
-        double qq[3] = {q[p], q[cap + p], q[2 * cap + p]};
+    while (j < total) {
         // filed site and ref_dist = modf(pos/dx) (Particles_gevolution.hpp:170 idiom)
         double rd[3];
         int ci[3];
@@ -540,11 +568,17 @@ __global__ __launch_bounds__(TPB) void k_update_tiled(
             q_out[js] = qq[0];
             q_out[cap + js] = qq[1];
             q_out[2 * cap + js] = qq[2];
-            id_out[js] = id_in[p];
+            id_out[js] = pid;
             const uint32_t c = (uint32_t)L(N, cell1<POW2>(x[0], dx, N), cell1<POW2>(x[1], dx, N),
                                            cell1<POW2>(x[2], dx, N));
             cell[js] = c;
-            slot[js] = atomicAdd(count + c, 1u);
+            atomicAdd(count + c, 1u);  // no return value: the wave does not wait for it
+        }
+        j += TPB;
+        if (j < total) {
+            js = sorted_index(j);
+            p = perm[js];
+            load_particle();
         }
     }
     if (KICK) {
@@ -572,12 +606,12 @@ void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
     if ((N & (N - 1)) == 0)
         hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, true>), dim3((unsigned)nblk), dim3(TPB), 0,
                            ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot,
+                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell,
                            ctx->pos2, ctx->q2, ctx->id, ctx->id2);
     else
         hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, false>), dim3((unsigned)nblk), dim3(TPB), 0,
                            ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->slot,
+                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell,
                            ctx->pos2, ctx->q2, ctx->id, ctx->id2);
 }
 
@@ -585,7 +619,8 @@ void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
 
 namespace gev {
 
-// scan count[] into start[] and build perm[] from cell[] / slot[]
+// scan count[] into start[] (count[] becomes the scatter cursor) and build
+// perm[] from cell[]
 static int finish_filing(gev_ctx *ctx) {
     const int64_t n3v = n3(ctx);
     const int64_t nblk = (n3v + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -600,7 +635,7 @@ static int finish_filing(gev_ctx *ctx) {
     if (ctx->n > 0) {
         ScopedTimer t(ctx, "filing_scatter");
         hipLaunchKernelGGL(k_scatter, dim3(grid_for(ctx->n, TPB)), dim3(TPB), 0, ctx->stream,
-                           ctx->cell, ctx->slot, ctx->start, ctx->n, ctx->perm);
+                           ctx->cell, ctx->count, ctx->n, ctx->perm);
     }
     {
         ScopedTimer t(ctx, "filing_sort");
@@ -617,7 +652,7 @@ int launch_refile(gev_ctx *ctx) {
     if (ctx->n > 0) {
         ScopedTimer t(ctx, "filing_count");
         hipLaunchKernelGGL(k_count_cells, dim3(grid_for(ctx->n, TPB)), dim3(TPB), 0, ctx->stream,
-                           ctx->pos, ctx->cap, ctx->n, ctx->N, ctx->count, ctx->cell, ctx->slot);
+                           ctx->pos, ctx->cap, ctx->n, ctx->N, ctx->count, ctx->cell);
     }
     rc = check_hip(ctx, hipGetLastError(), "count launch");
     if (rc) return rc;
