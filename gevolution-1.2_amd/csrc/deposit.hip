@@ -37,6 +37,9 @@ namespace {
 #ifndef GEV_DEP_XCD
 #define GEV_DEP_XCD 1
 #endif
+#ifndef GEV_DEP_ROWS
+#define GEV_DEP_ROWS 1
+#endif
 constexpr int TX = GEV_DEP_TX, TY = GEV_DEP_TY, TZ = GEV_DEP_TZ;
 constexpr int TPB = TX * TY * TZ;   // one target cell per thread
 constexpr int SX = TX + 1, SY = TY + 1, SZ = TZ + 1;
@@ -273,6 +276,27 @@ __device__ __forceinline__ void add_source_cell(Acc &A, const ST &st, const Targ
     add_source<KIND>(A, P, T, DX, DY, DZ, mass_s);
 }
 
+// the target's sums, written once
+template <int KIND>
+__device__ __forceinline__ void store_target(const Acc &A, double *__restrict__ out_s, double *__restrict__ out_t,
+                                             uint32_t t, int64_t n3v) {
+    using K = Kind<KIND>;
+    if (K::T00) out_s[t] = A.s00;
+    if (K::TIJ) {
+        out_t[t] = A.d0;
+        out_t[3 * n3v + t] = A.d1;
+        out_t[5 * n3v + t] = A.d2;
+        out_t[n3v + t] = A.o01;
+        out_t[2 * n3v + t] = A.o02;
+        out_t[4 * n3v + t] = A.o12;
+    }
+    if (K::T0I) {
+        out_t[t] = A.v0;
+        out_t[n3v + t] = A.v1;
+        out_t[2 * n3v + t] = A.v2;
+    }
+}
+
 // Runs CALL(dx, dy, dz) with the offset of a source from its target,
 // d = dx | dy << 1 | dz << 2, as compile-time values.  (A switch over
 // force-inlined template calls: every accumulator stays in registers.)
@@ -327,6 +351,141 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const bool interior = x0 > 0 && y0 > 0 && z0 > 0;
     const double dx = 1.0 / (double)N, Nd = (double)N;
     const int64_t n3v = (int64_t)N * N * N;
+
+    // 2. this thread's target (one per thread; the tile has TPB cells)
+    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+    const bool active = lx < nx && ly < ny && lz < nz;
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    Target T;
+    T.ph = T.plq01 = T.plq02 = T.plq12 = 0.;
+    T.lnk0 = T.lnk1 = T.lnk2 = 1.;
+    Acc A;
+    A.s00 = A.d0 = A.d1 = A.d2 = A.o01 = A.o02 = A.o12 = A.v0 = A.v1 = A.v2 = 0.;
+    const uint32_t t = active ? L(N, x, y, z) : 0u;
+    if (active) {
+        double phx = 0., phy = 0., phz = 0., phxy = 0., phxz = 0., phyz = 0.;
+        if (USE_PHI) {
+            const int xp = x == N - 1 ? 0 : x + 1, yp = y == N - 1 ? 0 : y + 1, zp = z == N - 1 ? 0 : z + 1;
+            T.ph = phi[t];
+            phx = phi[L(N, xp, y, z)];
+            phy = phi[L(N, x, yp, z)];
+            phz = phi[L(N, x, y, zp)];
+            phxy = phi[L(N, xp, yp, z)];
+            phxz = phi[L(N, xp, y, zp)];
+            phyz = phi[L(N, x, yp, zp)];
+        }
+        // plaquette sums (gevolution.hpp:1263-1272) and link factors (1129-1144)
+        T.plq01 = T.ph + phy + phx + phxy;
+        T.plq02 = T.ph + phz + phx + phxz;
+        T.plq12 = T.ph + phz + phy + phyz;
+        T.lnk0 = 1. + T.ph + phx;
+        T.lnk1 = 1. + T.ph + phy;
+        T.lnk2 = 1. + T.ph + phz;
+        if (ACCUM) {
+            if (K::T00) A.s00 = out_s[t];
+            if (K::TIJ) {
+                A.d0 = out_t[t];
+                A.d1 = out_t[3 * n3v + t];
+                A.d2 = out_t[5 * n3v + t];
+                A.o01 = out_t[n3v + t];
+                A.o02 = out_t[2 * n3v + t];
+                A.o12 = out_t[4 * n3v + t];
+            }
+            if (K::T0I) {
+                A.v0 = out_t[t];
+                A.v1 = out_t[n3v + t];
+                A.v2 = out_t[2 * n3v + t];
+            }
+        }
+    }
+    // Fast path: a full interior tile whose source region fits one chunk.
+    // Each of the region's SY x SZ rows (SX cells, ascending x at one y and
+    // z) is one contiguous run of the filing permutation, so the records
+    // follow from the row extents (read once per wavefront) without a
+    // scan, and a record's source cell is its particle's filed cell (the
+    // same integer as the region slot, so the same corner bits).  One
+    // barrier between the staging and the accumulation.
+    if (GEV_DEP_ROWS && FULL && interior) {
+        constexpr int NROW = SY * SZ;
+        static_assert(NROW <= 64, "one lane per row");
+        // lane i of every wavefront reads row i's extent (one vector load
+        // latency), then the extents are made wave-uniform
+        const int lane = threadIdx.x & 63;
+        uint32_t rb = 0, re = 0;
+        if (lane < NROW) {
+            const uint32_t g = L(N, x0 - 1, y0 - 1 + lane % SY, z0 - 1 + lane / SY);
+            rb = start[g];
+            re = start[g + SX];
+        }
+        uint32_t roff[NROW + 1], delta[NROW];  // record r of row i: permutation index r + delta[i]
+        roff[0] = 0;
+#pragma unroll
+        for (int i = 0; i < NROW; i++) {
+            const uint32_t b = __builtin_amdgcn_readlane(rb, i);
+            roff[i + 1] = roff[i] + (__builtin_amdgcn_readlane(re, i) - b);
+            delta[i] = b - roff[i];
+        }
+        const uint32_t total = roff[NROW];
+        if (total <= (uint32_t)CAP) {
+            constexpr int PR = (CAP + TPB - 1) / TPB;
+            uint32_t pidx[PR];
+#pragma unroll
+            for (int k = 0; k < PR; k++) {
+                const uint32_t r = threadIdx.x + k * TPB;
+                pidx[k] = 0;
+                if (r < total) {
+                    uint32_t d = delta[0];
+#pragma unroll
+                    for (int i = 1; i < NROW; i++) d = r >= roff[i] ? delta[i] : d;
+                    pidx[k] = perm[r + d];
+                }
+            }
+            // record offset and count of every source cell: row i's cells by thread i
+            if (threadIdx.x < NROW) {
+                const int i = threadIdx.x;
+                uint32_t d = delta[0];
+#pragma unroll
+                for (int k = 1; k < NROW; k++) d = i == k ? delta[k] : d;
+                const uint32_t g = L(N, x0 - 1, y0 - 1 + i % SY, z0 - 1 + i / SY);
+                uint32_t sv[SX + 1];
+#pragma unroll
+                for (int k = 0; k <= SX; k++) sv[k] = start[g + k];
+#pragma unroll
+                for (int k = 0; k < SX; k++) {
+                    s_off[i * SX + k] = sv[k] - d;
+                    s_cnt[i * SX + k] = sv[k + 1] - sv[k];
+                }
+            }
+            PQ pq[PR];
+#pragma unroll
+            for (int k = 0; k < PR; k++)
+                if (threadIdx.x + k * TPB < total) pq[k] = load_pq<KIND>(pos, q, cap, pidx[k]);
+#pragma unroll
+            for (int k = 0; k < PR; k++) {
+                const uint32_t r = threadIdx.x + k * TPB;
+                if (r < total) {
+                    const double rx = (int)floor(over_dx<POW2>(pq[k].x, dx, Nd)) * dx;
+                    const double ry = (int)floor(over_dx<POW2>(pq[k].y, dx, Nd)) * dx;
+                    const double rz = (int)floor(over_dx<POW2>(pq[k].z, dx, Nd)) * dx;
+                    st.put((int)r, make_rec<KIND, USE_PHI, POW2>(pq[k], rx, ry, rz, dx, Nd, a, mass_t));
+                }
+            }
+            __syncthreads();
+            if (active) {
+                const int cc = (lz * SY + ly) * SX + lx;  // source (dx, dy, dz) = (1, 1, 1)
+                add_source_cell<KIND, 1, 1, 1>(A, st, T, cc, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 1, 1>(A, st, T, cc + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 0, 1>(A, st, T, cc + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 0, 1>(A, st, T, cc + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 1, 0>(A, st, T, cc + SX * SY, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 1, 0>(A, st, T, cc + SX * SY + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 0, 0>(A, st, T, cc + SX * SY + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 0, 0>(A, st, T, cc + SX * SY + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                store_target<KIND>(A, out_s, out_t, t, n3v);
+            }
+            return;
+        }
+    }
 
     // 1. source cells in ascending site order: CSR begin and count (two
     //    cells per thread, all four loads in flight together)
@@ -385,52 +544,6 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
         __syncthreads();
     }
 
-    // 2. this thread's target (one per thread; the tile has TPB cells)
-    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
-    const bool active = lx < nx && ly < ny && lz < nz;
-    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
-    Target T;
-    T.ph = T.plq01 = T.plq02 = T.plq12 = 0.;
-    T.lnk0 = T.lnk1 = T.lnk2 = 1.;
-    Acc A;
-    A.s00 = A.d0 = A.d1 = A.d2 = A.o01 = A.o02 = A.o12 = A.v0 = A.v1 = A.v2 = 0.;
-    const uint32_t t = active ? L(N, x, y, z) : 0u;
-    if (active) {
-        double phx = 0., phy = 0., phz = 0., phxy = 0., phxz = 0., phyz = 0.;
-        if (USE_PHI) {
-            const int xp = x == N - 1 ? 0 : x + 1, yp = y == N - 1 ? 0 : y + 1, zp = z == N - 1 ? 0 : z + 1;
-            T.ph = phi[t];
-            phx = phi[L(N, xp, y, z)];
-            phy = phi[L(N, x, yp, z)];
-            phz = phi[L(N, x, y, zp)];
-            phxy = phi[L(N, xp, yp, z)];
-            phxz = phi[L(N, xp, y, zp)];
-            phyz = phi[L(N, x, yp, zp)];
-        }
-        // plaquette sums (gevolution.hpp:1263-1272) and link factors (1129-1144)
-        T.plq01 = T.ph + phy + phx + phxy;
-        T.plq02 = T.ph + phz + phx + phxz;
-        T.plq12 = T.ph + phz + phy + phyz;
-        T.lnk0 = 1. + T.ph + phx;
-        T.lnk1 = 1. + T.ph + phy;
-        T.lnk2 = 1. + T.ph + phz;
-        if (ACCUM) {
-            if (K::T00) A.s00 = out_s[t];
-            if (K::TIJ) {
-                A.d0 = out_t[t];
-                A.d1 = out_t[3 * n3v + t];
-                A.d2 = out_t[5 * n3v + t];
-                A.o01 = out_t[n3v + t];
-                A.o02 = out_t[2 * n3v + t];
-                A.o12 = out_t[4 * n3v + t];
-            }
-            if (K::T0I) {
-                A.v0 = out_t[t];
-                A.v1 = out_t[n3v + t];
-                A.v2 = out_t[2 * n3v + t];
-            }
-        }
-    }
     // 3. chunks of whole source cells, in site order (one chunk in the
     //    common case: the 425 source cells hold fewer than CAP particles)
     int cb = 0;
@@ -566,22 +679,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
         cb = ce;
     }
 
-    if (active) {
-        if (K::T00) out_s[t] = A.s00;
-        if (K::TIJ) {
-            out_t[t] = A.d0;
-            out_t[3 * n3v + t] = A.d1;
-            out_t[5 * n3v + t] = A.d2;
-            out_t[n3v + t] = A.o01;
-            out_t[2 * n3v + t] = A.o02;
-            out_t[4 * n3v + t] = A.o12;
-        }
-        if (K::T0I) {
-            out_t[t] = A.v0;
-            out_t[n3v + t] = A.v1;
-            out_t[2 * n3v + t] = A.v2;
-        }
-    }
+    if (active) store_target<KIND>(A, out_s, out_t, t, n3v);
 }
 
 template <int KIND, bool USE_PHI, bool ACCUM>

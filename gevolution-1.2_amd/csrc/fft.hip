@@ -40,7 +40,7 @@ __global__ __launch_bounds__(TPB) void k_r2c_x(const T *__restrict__ in,
     using C = typename cplx_of<T>::type;
     constexpr int M = N / 2;
     constexpr int LINES = x_lines<N>();
-    __shared__ C buf[LINES * M];
+    __shared__ C buf[LINES * xstride<M>()];
     __shared__ C tw[M];
     const int64_t row0 = (int64_t)blockIdx.x * LINES;
     for (int t = threadIdx.x; t < M; t += TPB) tw[t] = twN[2 * t];
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(TPB) void k_r2c_x(const T *__restrict__ in,
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int idx = threadIdx.x + k * TPB;
-            if (PER * TPB == LINES * M || idx < LINES * M) buf[idx] = v[k];
+            if (PER * TPB == LINES * M || idx < LINES * M) buf[eaddr<M, LINES, false>(idx / M, idx % M)] = v[k];
         }
     }
     __syncthreads();
@@ -69,8 +69,8 @@ __global__ __launch_bounds__(TPB) void k_r2c_x(const T *__restrict__ in,
         const int line = idx / (M + 1), k = idx % (M + 1);
         const int64_t row = row0 + line;
         if (row >= nrows) continue;
-        C A = buf[line * M + (k == M ? 0 : k)];
-        C B = cconj(buf[line * M + (k == 0 ? 0 : M - k)]);
+        C A = buf[eaddr<M, LINES, false>(line, k == M ? 0 : k)];
+        C B = cconj(buf[eaddr<M, LINES, false>(line, k == 0 ? 0 : M - k)]);
         C E = mk2<C>((A.x + B.x) * (T)0.5, (A.y + B.y) * (T)0.5);
         C D = mk2<C>((A.x - B.x) * (T)0.5, (A.y - B.y) * (T)0.5);
         C O = mul_mi(D);
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *
     using C = typename cplx_of<T>::type;
     constexpr int M = N / 2;
     constexpr int LINES = x_lines<N>();
-    __shared__ C buf[LINES * M];
+    __shared__ C buf[LINES * xstride<M>()];
     __shared__ C tw[M];
     const int64_t row0 = (int64_t)blockIdx.x * LINES;
     for (int t = threadIdx.x; t < M; t += TPB) tw[t] = twN[2 * t];
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *
             C S = cadd(Xk, cconj(Xm));
             C D = csub(Xk, cconj(Xm));
             C Z = cadd(S, mul_pi(cmul(cconj(wk[j]), D)));
-            buf[idx] = cconj(Z);  // inverse transform through the forward one
+            buf[eaddr<M, LINES, false>(idx / M, k)] = cconj(Z);  // inverse transform through the forward one
         }
     }
     __syncthreads();
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *
         const int line = idx / M, n = idx % M;
         const int64_t row = row0 + line;
         if (row >= nrows) continue;
-        C z = cconj(buf[idx]);
+        C z = cconj(buf[eaddr<M, LINES, false>(line, n)]);
         reinterpret_cast<C *>(out + row * N)[n] = z;
     }
 }

@@ -94,9 +94,13 @@ template <typename C> struct Butterfly<C, 3> {
 };
 
 // LDS element address: interleaved tiles (element e of line w at e*LINES+w)
-// or contiguous lines (line w at w*L).
+// or contiguous lines (line w from w*xstride<L>()).
+// Contiguous lines carry one padding element after every 16, so the
+// strided stores of the Stockham stages spread over the LDS banks; a line
+// then takes xstride<L>() elements.
+template <int L> constexpr int xstride() { return L + L / 16; }
 template <int L, int LINES, bool IL> __device__ __forceinline__ int eaddr(int line, int e) {
-    return IL ? e * LINES + line : line * L + e;
+    return IL ? e * LINES + line : line * xstride<L>() + e + e / 16;
 }
 
 // One Stockham stage of radix R on LINES transforms of length L held in LDS,

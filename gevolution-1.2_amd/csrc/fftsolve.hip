@@ -55,7 +55,11 @@ template <int N> constexpr int zs_cols() { return N >= 768 ? 2 : GEV_ZS_COLS; }
 // The tensor solve prefetches the carried B into registers during the last
 // input transform where the registers allow it; at N = 384 and above 512 the
 // prefetch would spill (compiler resource report), so B is read at the solve.
-template <int N> constexpr bool zs_prefetch_b() { return N <= 512 && N != 384; }
+#ifndef GEV_ZS_AHEAD
+#define GEV_ZS_AHEAD 1
+#endif
+constexpr int ZS_AHEAD = GEV_ZS_AHEAD;
+template <int N> constexpr bool zs_prefetch_b() { return ZS_AHEAD == 1 && N <= 512 && N != 384; }
 template <int KIND> constexpr int zs_pts() { return KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR ? GEV_ZS_PTS_TENSOR : 4; }
 template <int N, int KIND> constexpr int zs_threads() {
     int t = N * zs_cols<N>() / zs_pts<KIND>();
@@ -131,19 +135,23 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
     };
 
     double2 S[NIN][PTS];
-    double2 v[PTS];
-    load_comp(0, v);
+    // input components in flight ahead of the one being transformed: the
+    // next one, or the next two (ZS_AHEAD = 2, which reads B at the solve
+    // instead of prefetching it)
+    double2 v[ZS_AHEAD][PTS];
+#pragma unroll
+    for (int c = 0; c < ZS_AHEAD && c < NIN; c++) load_comp(c, v[c]);
 #pragma unroll
     for (int c = 0; c < NIN; c++) {
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
-            if (EXACT || idx < N * W) buf[idx] = v[j];
+            if (EXACT || idx < N * W) buf[idx] = v[c % ZS_AHEAD][j];
         }
-        // the next component's loads stay in flight through this component's
-        // transform (the stage barriers wait for LDS traffic only)
-        if (c + 1 < NIN) load_comp(c + 1, v);
-        else if (KIND == ZS_TENSOR && zs_prefetch_b<N>()) load_bold();
+        // the following components' loads stay in flight through this
+        // component's transform (the stage barriers wait for LDS traffic only)
+        if (c + ZS_AHEAD < NIN) load_comp(c + ZS_AHEAD, v[c % ZS_AHEAD]);
+        else if (c + 1 == NIN && KIND == ZS_TENSOR && zs_prefetch_b<N>()) load_bold();
         __syncthreads();
         stages<double2, N, W, true, 1, NT>(buf, tw);
         // each thread reads back only the points it wrote, so the next
@@ -272,7 +280,7 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
     constexpr int LINES = xprep_lines<N, NC>();
     constexpr int PER = (LINES * M + TPB - 1) / TPB;
     constexpr bool EXACT = PER * TPB == LINES * M;
-    __shared__ double2 buf[LINES * M];
+    __shared__ double2 buf[LINES * xstride<M>()];
     __shared__ double2 tw[M];
     const int64_t row0 = (int64_t)blockIdx.x * LINES;
     for (int t = threadIdx.x; t < M; t += TPB) tw[t] = A.tw[2 * t];
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int idx = threadIdx.x + k * TPB;
-            if (EXACT || idx < LINES * M) buf[idx] = v[q][k];
+            if (EXACT || idx < LINES * M) buf[eaddr<M, LINES, false>(idx / M, idx % M)] = v[q][k];
         }
         __syncthreads();
         stages<double2, M, LINES, false, 1>(buf, tw);
@@ -321,8 +329,8 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
             const int line = idx / (M + 1), k = idx % (M + 1);
             const int64_t row = row0 + line;
             if (row >= A.nrows) continue;
-            const double2 Ak = buf[line * M + (k == M ? 0 : k)];
-            const double2 Bk = cconj(buf[line * M + (k == 0 ? 0 : M - k)]);
+            const double2 Ak = buf[eaddr<M, LINES, false>(line, k == M ? 0 : k)];
+            const double2 Bk = cconj(buf[eaddr<M, LINES, false>(line, k == 0 ? 0 : M - k)]);
             const double2 E = c2((Ak.x + Bk.x) * 0.5, (Ak.y + Bk.y) * 0.5);
             const double2 D = c2((Ak.x - Bk.x) * 0.5, (Ak.y - Bk.y) * 0.5);
             const double2 O = mul_mi(D);

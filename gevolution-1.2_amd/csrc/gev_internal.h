@@ -63,7 +63,8 @@ struct gev_ctx {
     int64_t *id2 = nullptr;
     uint32_t *cell = nullptr;   // filed cell of every particle
     uint32_t *perm = nullptr;   // particle indices grouped by cell, ascending ID
-    uint32_t *count = nullptr;  // N^3 per-cell counts (the scatter cursor after the scan)
+    uint32_t *count = nullptr;  // N^3 per-cell counts; the scatter counts them back down to zero
+    bool count_clean = false;   // count is all zero (every completed filing leaves it so)
     uint32_t *start = nullptr;  // N^3 + 1 CSR offsets
     uint32_t *scan_part = nullptr;  // scan block partials
     int64_t scan_nblk = 0;

@@ -125,9 +125,7 @@ __global__ __launch_bounds__(TPB) void k_scan_partials(uint32_t *__restrict__ pa
         }
 }
 
-// writes the exclusive offsets to out (CSR start) and over the counts in
-// cnt, which the scatter then advances as per-cell cursors
-__global__ __launch_bounds__(TPB) void k_scan_final(uint32_t *__restrict__ cnt, int64_t n,
+__global__ __launch_bounds__(TPB) void k_scan_final(const uint32_t *__restrict__ cnt, int64_t n,
                                                     const uint32_t *__restrict__ part,
                                                     uint32_t *__restrict__ out) {
     const int64_t b0 = (int64_t)blockIdx.x * SCAN_BLOCK;
@@ -143,11 +141,10 @@ __global__ __launch_bounds__(TPB) void k_scan_final(uint32_t *__restrict__ cnt, 
         const uint4 o = make_uint4(off, off + v[k].x, off + v[k].x + v[k].y, off + v[k].x + v[k].y + v[k].z);
         if (i + 3 < n) {
             *reinterpret_cast<uint4 *>(out + i) = o;
-            *reinterpret_cast<uint4 *>(cnt + i) = o;
         } else {
-            if (i < n) out[i] = cnt[i] = o.x;
-            if (i + 1 < n) out[i + 1] = cnt[i + 1] = o.y;
-            if (i + 2 < n) out[i + 2] = cnt[i + 2] = o.z;
+            if (i < n) out[i] = o.x;
+            if (i + 1 < n) out[i + 1] = o.y;
+            if (i + 2 < n) out[i + 2] = o.z;
         }
         carry += total;
     }
@@ -155,13 +152,18 @@ __global__ __launch_bounds__(TPB) void k_scan_final(uint32_t *__restrict__ cnt, 
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry;
 }
 
-// every particle takes the next place of its cell (the in-cell order is
-// then fixed by the ID sort)
-__global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ cell, uint32_t *__restrict__ cursor,
-                                                 int64_t n, uint32_t *__restrict__ perm) {
+// every particle takes a place of its cell, counting the cell's count back
+// down (so every completed filing leaves count[] all zero for the next one);
+// the in-cell order is then fixed by the ID sort.  A place outside the
+// cell's CSR range (counts not zero before the counting pass) is dropped.
+__global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ cell, const uint32_t *__restrict__ start,
+                                                 uint32_t *__restrict__ count, int64_t n,
+                                                 uint32_t *__restrict__ perm) {
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= n) return;
-    perm[atomicAdd(cursor + cell[p], 1u)] = (uint32_t)p;
+    const uint32_t c = cell[p];
+    const uint32_t b = start[c], k = atomicSub(count + c, 1u);
+    if (k - 1u < start[c + 1] - b) perm[b + k - 1u] = (uint32_t)p;
 }
 
 // ascending ID inside every cell (insertion sort; cells hold few particles)
@@ -444,21 +446,26 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
     const bool need_chi = (KICK && sa.nfk >= 2) || (DRIFT && PM == 0 && sa.nfd >= 2);
     const bool need_B = (KICK && QM == 0 && sa.nfk >= 3) || (DRIFT && PM == 0 && sa.nfd >= 3);
     // rows of the tile: cells (x0 .. x0+nx-1, y, z) are one run of the
-    // permutation.  Their extents are wave-uniform (scalar loads), so every
+    // permutation.  Lane r of every wavefront reads row r's extent (one
+    // vector load latency) and the extents are made wave-uniform, so every
     // wave finds its particles without waiting for the field staging.
+    static_assert(KTY * KTZ <= 64, "one lane per row");
+    uint32_t rb = 0, re = 0;
+    {
+        const int r = threadIdx.x & 63, ly = r % KTY, lz = r / KTY;
+        if (r < KTY * KTZ && ly < ny && lz < nz) {
+            const int64_t g0 = L(N, x0, y0 + ly, z0 + lz);
+            rb = start[g0];
+            re = start[g0 + nx];
+        }
+    }
     uint32_t rowbeg[KTY * KTZ], rowoff[KTY * KTZ + 1];
     rowoff[0] = 0;
 #pragma unroll
     for (int r = 0; r < KTY * KTZ; r++) {
-        const int ly = r % KTY, lz = r / KTY;
-        uint32_t b = 0, e = 0;
-        if (ly < ny && lz < nz) {
-            const int64_t g0 = L(N, x0, y0 + ly, z0 + lz);
-            b = start[g0];
-            e = start[g0 + nx];
-        }
+        const uint32_t b = __builtin_amdgcn_readlane(rb, r);
         rowbeg[r] = b;
-        rowoff[r + 1] = rowoff[r] + (e - b);
+        rowoff[r + 1] = rowoff[r] + (__builtin_amdgcn_readlane(re, r) - b);
     }
     const uint32_t total = rowoff[KTY * KTZ];
     // cell-sorted index of the tile's j-th particle
@@ -619,8 +626,7 @@ void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
 
 namespace gev {
 
-// scan count[] into start[] (count[] becomes the scatter cursor) and build
-// perm[] from cell[]
+// scan count[] into start[] and build perm[] from cell[] (count[] ends zero)
 static int finish_filing(gev_ctx *ctx) {
     const int64_t n3v = n3(ctx);
     const int64_t nblk = (n3v + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -635,20 +641,23 @@ static int finish_filing(gev_ctx *ctx) {
     if (ctx->n > 0) {
         ScopedTimer t(ctx, "filing_scatter");
         hipLaunchKernelGGL(k_scatter, dim3(grid_for(ctx->n, TPB)), dim3(TPB), 0, ctx->stream,
-                           ctx->cell, ctx->count, ctx->n, ctx->perm);
+                           ctx->cell, ctx->start, ctx->count, ctx->n, ctx->perm);
     }
     {
         ScopedTimer t(ctx, "filing_sort");
         hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(n3v, TPB)), dim3(TPB), 0, ctx->stream,
                            ctx->start, ctx->id, n3v, ctx->perm);
     }
-    return check_hip(ctx, hipGetLastError(), "filing launch");
+    const int rc = check_hip(ctx, hipGetLastError(), "filing launch");
+    if (!rc) ctx->count_clean = true;  // the scatter counted every cell back to zero
+    return rc;
 }
 
 int launch_refile(gev_ctx *ctx) {
     int rc = check_hip(ctx, hipMemsetAsync(ctx->count, 0, sizeof(uint32_t) * n3(ctx), ctx->stream),
                        "count clear");
     if (rc) return rc;
+    ctx->count_clean = false;
     if (ctx->n > 0) {
         ScopedTimer t(ctx, "filing_count");
         hipLaunchKernelGGL(k_count_cells, dim3(grid_for(ctx->n, TPB)), dim3(TPB), 0, ctx->stream,
@@ -677,11 +686,14 @@ int launch_particle_update(gev_ctx *ctx, bool kick, int qm, double dtk, int nfk,
                        "vmax clear");
         if (rc) return rc;
     }
-    if (drift) {
+    if (drift && !ctx->count_clean) {
+        // a filing that did not complete left counts behind
         rc = check_hip(ctx, hipMemsetAsync(ctx->count, 0, sizeof(uint32_t) * n3(ctx), ctx->stream),
                        "count clear");
         if (rc) return rc;
+        ctx->count_clean = true;
     }
+    if (drift) ctx->count_clean = false;
     if (ctx->n > 0) {
         const char *name = kick && drift ? "kick_drift" : (kick ? "update_vel" : "move_particles");
         ScopedTimer t(ctx, name);
