@@ -186,7 +186,7 @@ def main():
     ap.add_argument("--boxsize", type=float, default=None, help="Mpc/h (default 4 * ngrid)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ngrid", type=int, default=256)
-    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--cpu-cycles", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-timers", action="store_true", help="time without per-kernel events")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_gr_cycle_512_fp64.txt"),
