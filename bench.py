@@ -73,6 +73,11 @@ def algorithmic_bytes(name, N, Np):
         "filing_scatter": 12 * Np,
         "filing_sort": 4 * n3,
     }
+    # a pass launched over n components at once is timed as "name[n]"
+    if name.endswith("]") and "[" in name:
+        base, n = name[:-1].split("[")
+        b = table.get(base)
+        return None if b is None else b * int(n)
     return table.get(name)
 
 

@@ -36,8 +36,11 @@ template <typename T, int N>
 __global__ __launch_bounds__(TPB) void k_r2c_x(const T *__restrict__ in,
                                                typename cplx_of<T>::type *__restrict__ out, int P,
                                                int64_t nrows,
-                                               const typename cplx_of<T>::type *__restrict__ twN) {
+                                               const typename cplx_of<T>::type *__restrict__ twN,
+                                               int64_t in_cs, int64_t out_cs) {
     using C = typename cplx_of<T>::type;
+    in += blockIdx.z * in_cs;  // component blockIdx.z
+    out += blockIdx.z * out_cs;
     constexpr int M = N / 2;
     constexpr int LINES = x_lines<N>();
     __shared__ C buf[LINES * xstride<M>()];
@@ -83,8 +86,11 @@ __global__ __launch_bounds__(TPB) void k_r2c_x(const T *__restrict__ in,
 template <typename T, int N>
 __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *__restrict__ in,
                                                T *__restrict__ out, int P, int64_t nrows,
-                                               const typename cplx_of<T>::type *__restrict__ twN) {
+                                               const typename cplx_of<T>::type *__restrict__ twN,
+                                               int64_t in_cs, int64_t out_cs) {
     using C = typename cplx_of<T>::type;
+    in += blockIdx.z * in_cs;  // component blockIdx.z
+    out += blockIdx.z * out_cs;
     constexpr int M = N / 2;
     constexpr int LINES = x_lines<N>();
     __shared__ C buf[LINES * xstride<M>()];
@@ -140,8 +146,11 @@ template <typename T, int N, bool INV>
 __global__ __launch_bounds__(TPB) void k_c2c_cols(const typename cplx_of<T>::type *in,
                                                   typename cplx_of<T>::type *out, int H,
                                                   int64_t es, int64_t os,
-                                                  const typename cplx_of<T>::type *__restrict__ twN) {
+                                                  const typename cplx_of<T>::type *__restrict__ twN,
+                                                  int64_t cs) {
     using C = typename cplx_of<T>::type;
+    in += blockIdx.z * cs;  // component blockIdx.z
+    out += blockIdx.z * cs;
     constexpr int W = col_tile<T, N>();
     __shared__ C buf[N * W];
     __shared__ C tw[N];
@@ -178,7 +187,21 @@ __global__ __launch_bounds__(TPB) void k_c2c_cols(const typename cplx_of<T>::typ
     }
 }
 
-// passes: bit 0 = x (r2c), bit 1 = y, bit 2 = z
+// Timer names of a pass launched over n components (n <= MAXC): the plain
+// name for one, name[n] for several (the per-launch byte count scales by n).
+constexpr int MAXC = 6;
+enum { P_R2C_X, P_C2C_Y, P_C2C_Z, P_C2C_Z_INV, P_C2C_Y_INV, P_C2R_X };
+const char *pass_name(int pass, int n) {
+#define GEV_NAMES(s) {s, s "[2]", s "[3]", s "[4]", s "[5]", s "[6]"}
+    static const char *const names[6][MAXC] = {GEV_NAMES("fft_r2c_x"),     GEV_NAMES("fft_c2c_y"),
+                                               GEV_NAMES("fft_c2c_z"),     GEV_NAMES("fft_c2c_z_inv"),
+                                               GEV_NAMES("fft_c2c_y_inv"), GEV_NAMES("fft_c2r_x")};
+#undef GEV_NAMES
+    return names[pass][n - 1];
+}
+
+// passes: bit 0 = x (r2c), bit 1 = y, bit 2 = z, all components pass by pass.
+// The x pass is one launch over up to MAXC components (blockIdx.z).
 template <typename T, int N>
 int run_forward(gev_ctx *ctx, const T *real, typename cplx_of<T>::type *k, int ncomp,
                 const typename cplx_of<T>::type *tw, int passes) {
@@ -187,30 +210,32 @@ int run_forward(gev_ctx *ctx, const T *real, typename cplx_of<T>::type *k, int n
     const int64_t n3v = (int64_t)N * N * N, nkv = (int64_t)P * N * N;
     constexpr int LINES = x_lines<N>();
     constexpr int W = col_tile<T, N>();
-    for (int c = 0; c < ncomp; c++) {
+    for (int c0 = 0; c0 < ncomp; c0 += MAXC) {
+        const int nc = ncomp - c0 < MAXC ? ncomp - c0 : MAXC;
         if (passes & 1) {
-            gev::ScopedTimer t(ctx, "fft_r2c_x");
-            hipLaunchKernelGGL((k_r2c_x<T, N>), dim3(gev::grid_for(nrows, LINES)), dim3(TPB), 0,
-                               ctx->stream, real + c * n3v, k + c * nkv, P, nrows, tw);
+            gev::ScopedTimer t(ctx, pass_name(P_R2C_X, nc));
+            hipLaunchKernelGGL((k_r2c_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nc), dim3(TPB), 0,
+                               ctx->stream, real + c0 * n3v, k + c0 * nkv, P, nrows, tw, n3v, nkv);
         }
-        if (passes & 2) {
-            gev::ScopedTimer t(ctx, "fft_c2c_y");
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(TPB), 0,
-                               ctx->stream, k + c * nkv, k + c * nkv, H, (int64_t)P,
-                               (int64_t)P * N, tw);
+        // the column passes run one launch per component (measured faster
+        // than one launch over the components)
+        for (int c = c0; c < c0 + nc && (passes & 2); c++) {
+            gev::ScopedTimer t(ctx, pass_name(P_C2C_Y, 1));
+            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(TPB), 0, ctx->stream,
+                               k + c * nkv, k + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
         }
-        if (passes & 4) {
-            gev::ScopedTimer t(ctx, "fft_c2c_z");
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(TPB), 0,
-                               ctx->stream, k + c * nkv, k + c * nkv, H, (int64_t)P * N,
-                               (int64_t)P, tw);
+        for (int c = c0; c < c0 + nc && (passes & 4); c++) {
+            gev::ScopedTimer t(ctx, pass_name(P_C2C_Z, 1));
+            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(TPB), 0, ctx->stream,
+                               k + c * nkv, k + c * nkv, H, (int64_t)P * N, (int64_t)P, tw, (int64_t)0);
         }
     }
     return gev::check_hip(ctx, hipGetLastError(), "fft forward launch");
 }
 
 // from_y: k already went through the inverse z pass; the remaining passes run
-// in place on k itself (tmp unused)
+// in place on k itself (tmp unused), the x pass as one launch over up to MAXC
+// components.  Otherwise every component goes z, y, x through tmp.
 template <typename T, int N>
 int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int ncomp,
                  typename cplx_of<T>::type *tmp, const typename cplx_of<T>::type *tw, bool from_y) {
@@ -219,23 +244,38 @@ int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int 
     const int64_t n3v = (int64_t)N * N * N, nkv = (int64_t)P * N * N;
     constexpr int LINES = x_lines<N>();
     constexpr int W = col_tile<T, N>();
+    if (from_y) {
+        auto *kk = const_cast<typename cplx_of<T>::type *>(k);
+        for (int c0 = 0; c0 < ncomp; c0 += MAXC) {
+            const int nc = ncomp - c0 < MAXC ? ncomp - c0 : MAXC;
+            for (int c = c0; c < c0 + nc; c++) {
+                gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
+                hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(TPB), 0, ctx->stream,
+                                   kk + c * nkv, kk + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
+            }
+            {
+                gev::ScopedTimer t(ctx, pass_name(P_C2R_X, nc));
+                hipLaunchKernelGGL((k_c2r_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nc), dim3(TPB), 0,
+                                   ctx->stream, kk + c0 * nkv, real + c0 * n3v, P, nrows, tw, nkv, n3v);
+            }
+        }
+        return gev::check_hip(ctx, hipGetLastError(), "fft backward launch");
+    }
     for (int c = 0; c < ncomp; c++) {
-        if (from_y) {
-            tmp = const_cast<typename cplx_of<T>::type *>(k + c * nkv);
-        } else {
-            gev::ScopedTimer t(ctx, "fft_c2c_z_inv");
+        {
+            gev::ScopedTimer t(ctx, pass_name(P_C2C_Z_INV, 1));
             hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(TPB), 0,
-                               ctx->stream, k + c * nkv, tmp, H, (int64_t)P * N, (int64_t)P, tw);
+                               ctx->stream, k + c * nkv, tmp, H, (int64_t)P * N, (int64_t)P, tw, (int64_t)0);
         }
         {
-            gev::ScopedTimer t(ctx, "fft_c2c_y_inv");
+            gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
             hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(TPB), 0,
-                               ctx->stream, tmp, tmp, H, (int64_t)P, (int64_t)P * N, tw);
+                               ctx->stream, tmp, tmp, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
         }
         {
-            gev::ScopedTimer t(ctx, "fft_c2r_x");
+            gev::ScopedTimer t(ctx, pass_name(P_C2R_X, 1));
             hipLaunchKernelGGL((k_c2r_x<T, N>), dim3(gev::grid_for(nrows, LINES)), dim3(TPB), 0,
-                               ctx->stream, tmp, real + c * n3v, P, nrows, tw);
+                               ctx->stream, tmp, real + c * n3v, P, nrows, tw, (int64_t)0, (int64_t)0);
         }
     }
     return gev::check_hip(ctx, hipGetLastError(), "fft backward launch");

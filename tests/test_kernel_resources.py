@@ -33,7 +33,7 @@ def test_no_kernel_uses_scratch(rows):
 
 
 @pytest.mark.parametrize("mangled,waves", [
-    ("k_update_tiledILb1ELi0ELb1ELi0ELb1E", 4),        # kick + drift, GR, power-of-two N
+    ("k_update_tiledILb1ELi0ELb1ELi0ELb1E", 3),        # kick + drift, GR, power-of-two N (49 KB LDS)
     ("k_deposit_tiledILi3ELb1ELb0ELb1ELb1E", 3),       # T00 + Tij, full tiles
     ("k_r2c_x_prepILi512ELi6E", 3),                    # tensor prepare + x pass
     ("k_zsolveILi512ELi1E", 2),                        # tensor z solve
