@@ -261,11 +261,12 @@ template <int KIND> __device__ __forceinline__ bool feeds(int dxx, int dy, int d
 // (DX, DY, DZ) fixed at compile time (same operations, same order as the
 // reference's local cube; the conditional terms of add_particle / add_source
 // fold away).  c: the source cell's index in the tile's source region.
-template <int KIND, int DX, int DY, int DZ, typename ST>
+// RANGE = false: the caller knows c is in the chunk [cb, ce) (one chunk).
+template <int KIND, int DX, int DY, int DZ, bool RANGE = true, typename ST>
 __device__ __forceinline__ void add_source_cell(Acc &A, const ST &st, const Target &T, int c, int cb, int ce,
                                                 uint32_t base, const uint32_t *s_off, const uint32_t *s_cnt,
                                                 double mass_s) {
-    if (!feeds<KIND>(DX, DY, DZ) || c < cb || c >= ce) return;
+    if (!feeds<KIND>(DX, DY, DZ) || (RANGE && (c < cb || c >= ce))) return;
     const uint32_t n = s_cnt[c];
     if (n == 0) return;
     Partial P;
@@ -473,14 +474,14 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             __syncthreads();
             if (active) {
                 const int cc = (lz * SY + ly) * SX + lx;  // source (dx, dy, dz) = (1, 1, 1)
-                add_source_cell<KIND, 1, 1, 1>(A, st, T, cc, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 1, 1>(A, st, T, cc + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 0, 1>(A, st, T, cc + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 0, 1>(A, st, T, cc + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 1, 0>(A, st, T, cc + SX * SY, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 1, 0>(A, st, T, cc + SX * SY + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 0, 0>(A, st, T, cc + SX * SY + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 0, 0>(A, st, T, cc + SX * SY + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 1, 1, false>(A, st, T, cc, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 1, 1, false>(A, st, T, cc + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 0, 1, false>(A, st, T, cc + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 0, 1, false>(A, st, T, cc + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 1, 0, false>(A, st, T, cc + SX * SY, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 1, 0, false>(A, st, T, cc + SX * SY + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 0, 0, false>(A, st, T, cc + SX * SY + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 0, 0, 0, false>(A, st, T, cc + SX * SY + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
                 store_target<KIND>(A, out_s, out_t, t, n3v);
             }
             return;

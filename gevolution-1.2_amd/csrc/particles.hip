@@ -190,10 +190,10 @@ __global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__
 
 // Particles are updated tile by tile: a workgroup takes the particles filed
 // in a KTX x KTY x KTZ block of cells (default 32x4x4; runs of the filing
-// permutation) and first stages phi, chi and B over the block plus a
-// one-cell halo in LDS, so every gather of update_q / update_pos is an LDS
-// read.  The long x extent keeps the halo's share of the staged cells down
-// (2.4x the block against 2.5x for 16x4x4; 16x8x4 measured slower).
+// permutation) and first stages phi, chi and B in LDS, each over the block
+// plus the halo its stencils reach (Region below), so every gather of
+// update_q / update_pos is an LDS read.  The long x extent keeps the halo's
+// share of the staged cells down (16x4x4 and 16x8x4 measured slower).
 #ifndef GEV_KICK_TX
 #define GEV_KICK_TX 32
 #endif
@@ -204,17 +204,34 @@ __global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__
 #define GEV_KICK_TZ 4
 #endif
 constexpr int KTX = GEV_KICK_TX, KTY = GEV_KICK_TY, KTZ = GEV_KICK_TZ;
-constexpr int FX = KTX + 2, FY = KTY + 2, FZ = KTZ + 2;
-constexpr int FREG = FX * FY * FZ;  // staged cells per field (1224 for 32x4x4)
 
-// one field around the particle's cell: (dx, dy, dz) in {-1, 0, 1}^3
-struct FieldView {
-    const double *f;  // LDS image of the field over the staged region
+// The staged region of one field: the tile's cells plus the cell above on
+// every axis (the stencils reach offsets 0 and 1), and the cell below on
+// axis LO only: the gradients of update_q reach offset -1 in x for B0, in y
+// for B1 and in z for B2, never for phi or chi (LO = -1).
+template <int LO> struct Region {
+    static constexpr int OX = LO == 0, OY = LO == 1, OZ = LO == 2;  // lower halo per axis
+    static constexpr int SX = KTX + 1 + OX, SY = KTY + 1 + OY, SZ = KTZ + 1 + OZ;
+    static constexpr int SIZE = SX * SY * SZ;
+};
+
+// one field around the particle's cell: (dx, dy, dz) within the region
+template <int LO> struct FieldView {
+    const double *f;  // LDS image of the field over its staged region
     int base;         // region index of the particle's cell
     __device__ __forceinline__ double operator()(int dx, int dy, int dz) const {
-        return f[base + dx + FX * (dy + FY * dz)];
+        return f[base + dx + Region<LO>::SX * (dy + Region<LO>::SY * dz)];
     }
 };
+template <int LO> __device__ __forceinline__ FieldView<LO> view(const double *f, const int c[3], int x0, int y0,
+                                                                 int z0) {
+    using R = Region<LO>;
+    return FieldView<LO>{f, (c[0] - x0 + R::OX) + R::SX * ((c[1] - y0 + R::OY) + R::SY * (c[2] - z0 + R::OZ))};
+}
+using ViewS = FieldView<-1>;  // phi, chi
+using View0 = FieldView<0>;
+using View1 = FieldView<1>;
+using View2 = FieldView<2>;
 
 // Compiler-only memory fence between the phases of the particle update: the
 // LDS field values of one phase are read again in the next rather than held
@@ -223,8 +240,8 @@ struct FieldView {
 
 // update_q, gevolution.hpp:570-678 (GRADIENT_ORDER 1); returns v^2/a^2
 template <bool POW2>
-__device__ __forceinline__ double update_q(double q[3], const double rd[3], FieldView phi,
-                                           FieldView chi, FieldView B0, FieldView B1, FieldView B2,
+__device__ __forceinline__ double update_q(double q[3], const double rd[3], ViewS phi,
+                                           ViewS chi, View0 B0, View1 B1, View2 B2,
                                            int nfield, const double params[2], double dtau,
                                            double dx) {
     double gradphi[3], pgradB[3];
@@ -308,8 +325,8 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], Fiel
 
 // update_q_Newton, gevolution.hpp:709-776
 template <bool POW2>
-__device__ __forceinline__ double update_q_newton(double q[3], const double rd[3], FieldView psi,
-                                                  FieldView chi, int nfield, const double params[2],
+__device__ __forceinline__ double update_q_newton(double q[3], const double rd[3], ViewS psi,
+                                                  ViewS chi, int nfield, const double params[2],
                                                   double dtau, double dx) {
     double g[3];
     g[0] = (1. - rd[1]) * (1. - rd[2]) * (psi(1, 0, 0) - psi(0, 0, 0));
@@ -349,8 +366,8 @@ __device__ __forceinline__ double update_q_newton(double q[3], const double rd[3
 
 // update_pos, gevolution.hpp:810-871
 __device__ __forceinline__ void update_pos(double pos[3], const double q[3], const double rd[3],
-                                           FieldView phi, FieldView chi, FieldView B0,
-                                           FieldView B1, FieldView B2, int nfield,
+                                           ViewS phi, ViewS chi, View0 B0,
+                                           View1 B1, View2 B2, int nfield,
                                            const double params[2], double dtau) {
     double v[3];
     double v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
@@ -441,7 +458,8 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
     unsigned long long *__restrict__ vmax, uint32_t *__restrict__ count, uint32_t *__restrict__ cell,
     double *__restrict__ pos_out, double *__restrict__ q_out,
     const int64_t *__restrict__ id_in, int64_t *__restrict__ id_out) {
-    __shared__ double sf[5][FREG];
+    __shared__ double s_phi[Region<-1>::SIZE], s_chi[Region<-1>::SIZE];
+    __shared__ double s_b0[Region<0>::SIZE], s_b1[Region<1>::SIZE], s_b2[Region<2>::SIZE];
     __shared__ unsigned long long sh[TPB / 64];
 
     const int64_t nblk = (int64_t)ntx * nty * ntz;
@@ -501,21 +519,39 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
         jsn = sorted_index(j + TPB);
         pn = perm[jsn];
     }
-    constexpr int PER = (FREG + TPB - 1) / TPB;
-    double v[5][PER];
+    // field staging: element k of this thread's share of field f's region
+    auto cell_of = [&](auto reg, int i) {
+        using R = decltype(reg);
+        const int rx = i % R::SX, ry = (i / R::SX) % R::SY, rz = i / (R::SX * R::SY);
+        return L(N, wrapi(x0 - R::OX + rx, N), wrapi(y0 - R::OY + ry, N), wrapi(z0 - R::OZ + rz, N));
+    };
+    constexpr int PS = (Region<-1>::SIZE + TPB - 1) / TPB, P0 = (Region<0>::SIZE + TPB - 1) / TPB;
+    constexpr int P1 = (Region<1>::SIZE + TPB - 1) / TPB, P2 = (Region<2>::SIZE + TPB - 1) / TPB;
+    double vp[PS], vc[PS], v0[P0], v1[P1], v2[P2];
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
+    for (int k = 0; k < PS; k++) {
         const int i = threadIdx.x + k * TPB;
-        if (i < FREG) {
-            const int rx = i % FX, ry = (i / FX) % FY, rz = i / (FX * FY);
-            const int64_t g = L(N, wrapi(x0 - 1 + rx, N), wrapi(y0 - 1 + ry, N), wrapi(z0 - 1 + rz, N));
-            if (need_phi) v[0][k] = phi[g];
-            if (need_chi) v[1][k] = chi[g];
-            if (need_B) {
-                v[2][k] = Bi[g];
-                v[3][k] = Bi[n3v + g];
-                v[4][k] = Bi[2 * n3v + g];
-            }
+        if (i < Region<-1>::SIZE) {
+            const int64_t g = cell_of(Region<-1>{}, i);
+            if (need_phi) vp[k] = phi[g];
+            if (need_chi) vc[k] = chi[g];
+        }
+    }
+    if (need_B) {
+#pragma unroll
+        for (int k = 0; k < P0; k++) {
+            const int i = threadIdx.x + k * TPB;
+            if (i < Region<0>::SIZE) v0[k] = Bi[cell_of(Region<0>{}, i)];
+        }
+#pragma unroll
+        for (int k = 0; k < P1; k++) {
+            const int i = threadIdx.x + k * TPB;
+            if (i < Region<1>::SIZE) v1[k] = Bi[n3v + cell_of(Region<1>{}, i)];
+        }
+#pragma unroll
+        for (int k = 0; k < P2; k++) {
+            const int i = threadIdx.x + k * TPB;
+            if (i < Region<2>::SIZE) v2[k] = Bi[2 * n3v + cell_of(Region<2>{}, i)];
         }
     }
     double x[3] = {0., 0., 0.}, qq[3] = {0., 0., 0.}, xn[3] = {0., 0., 0.}, qn[3] = {0., 0., 0.};
@@ -532,17 +568,23 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
     if (j < total) load_particle(p, x, qq, pid);
     if (j + TPB < total) load_particle(pn, xn, qn, pidn);
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
+    for (int k = 0; k < PS; k++) {
         const int i = threadIdx.x + k * TPB;
-        if (i < FREG) {
-            if (need_phi) sf[0][i] = v[0][k];
-            if (need_chi) sf[1][i] = v[1][k];
-            if (need_B) {
-                sf[2][i] = v[2][k];
-                sf[3][i] = v[3][k];
-                sf[4][i] = v[4][k];
-            }
+        if (i < Region<-1>::SIZE) {
+            if (need_phi) s_phi[i] = vp[k];
+            if (need_chi) s_chi[i] = vc[k];
         }
+    }
+    if (need_B) {
+#pragma unroll
+        for (int k = 0; k < P0; k++)
+            if (threadIdx.x + k * TPB < Region<0>::SIZE) s_b0[threadIdx.x + k * TPB] = v0[k];
+#pragma unroll
+        for (int k = 0; k < P1; k++)
+            if (threadIdx.x + k * TPB < Region<1>::SIZE) s_b1[threadIdx.x + k * TPB] = v1[k];
+#pragma unroll
+        for (int k = 0; k < P2; k++)
+            if (threadIdx.x + k * TPB < Region<2>::SIZE) s_b2[threadIdx.x + k * TPB] = v2[k];
     }
     __syncthreads();
 
@@ -558,9 +600,10 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
             ci[i] = (int)fl;
             rd[i] = t - fl;
         }
-        const int base = (ci[0] - x0 + 1) + FX * ((ci[1] - y0 + 1) + FY * (ci[2] - z0 + 1));
-        const FieldView fphi{sf[0], base}, fchi{sf[1], base};
-        const FieldView fb0{sf[2], base}, fb1{sf[3], base}, fb2{sf[4], base};
+        const ViewS fphi = view<-1>(s_phi, ci, x0, y0, z0), fchi = view<-1>(s_chi, ci, x0, y0, z0);
+        const View0 fb0 = view<0>(s_b0, ci, x0, y0, z0);
+        const View1 fb1 = view<1>(s_b1, ci, x0, y0, z0);
+        const View2 fb2 = view<2>(s_b2, ci, x0, y0, z0);
         if (KICK) {
             double r;
             if (QM == 0)
