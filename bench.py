@@ -200,6 +200,61 @@ def cpu_baseline(N_sample, cycles, threads):
     return N ** 3 * cycles / dt, dt
 
 
+def timed_cycles(box, sim, st, warmup, steps, timers):
+    """warmup untimed cycles, then steps timed ones; (seconds, per-kernel table)."""
+    box.run_cycles(sim, st, warmup)
+    box.synchronize()
+    if timers:
+        box.timing_reset()
+        box.timing(True)
+    t0 = time.perf_counter()
+    box.run_cycles(sim, st, steps)
+    box.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernels = box.timing_report() if timers else {}
+    box.timing(False)
+    return elapsed, kernels
+
+
+def delta_spectrum(gev, box, a, numbins):
+    """The matter power spectrum of output.hpp:1990-2028: T00 with phi, FFT,
+    extractPowerSpectrum (deconvolved, linear k in units of 2 pi / box)."""
+    box.projection_init(gev.SOURCE)
+    box.projection_T00_project(a, True, 1.0)
+    box.fft(gev.PLAN_SOURCE, gev.FFT_FORWARD)
+    return box.extractPowerSpectrum(gev.SCALARFT, numbins)
+
+
+def precision_form_report(gev, box, st, box32, st32, sim, args, ms64):
+    """Times the FP32 form over the same cycles from the same initial state
+    and states its error against the FP64 form: phi (max norm) and the
+    matter power spectrum below k_Nyq/2."""
+    N = box.N
+    el, k32 = timed_cycles(box32, sim, st32, args.warmup, args.steps, not args.no_timers)
+    ms32 = 1e3 * el / args.steps
+    assert st32.a == st.a
+    phi64 = box.download_field(gev.PHI)
+    phi32 = box32.download_field(gev.PHI)
+    phi_err = float(np.max(np.abs(phi32 - phi64)) / np.max(np.abs(phi64)))
+    del phi64, phi32
+    nb = N // 2
+    p64 = delta_spectrum(gev, box, st.a, nb)
+    p32 = delta_spectrum(gev, box32, st32.a, nb)
+    low = (p64[4] > 0) & (p64[0] < 0.5 * np.pi * N)
+    pk_err = float(np.max(np.abs(p32[1][low] - p64[1][low]) / np.abs(p64[1][low])))
+    top = sorted(k32.items(), key=lambda kv: -kv[1][1])[:8]
+    print("FP32 form: %.3f ms/cycle (%.2fx), phi error %.2e, P(k) error %.2e"
+          % (ms32, ms64 / ms32, phi_err, pk_err), file=sys.stderr, flush=True)
+    return {"form": "fp32 fields and FFT passes; fp64 particles, background, deposit sums, stencil and "
+                    "k-space solve arithmetic (k->0 and zero mode)",
+            "ms_per_step": round(ms32, 3), "value": round(box.N ** 3 * args.steps / el, 1),
+            "speedup_vs_fp64": round(ms64 / ms32, 3),
+            "phi_max_rel_err": phi_err, "pk_max_rel_err_below_half_nyquist": pk_err,
+            "pk_bins_compared": int(low.sum()), "cycles_from_same_state": args.warmup + args.steps,
+            "z_at_comparison": round(1.0 / st.a - 1.0, 4),
+            "kernels_ms_per_launch": {n: round(ms / c, 4) for n, (c, ms) in top}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +272,8 @@ def main():
     ap.add_argument("--state", choices=["ic", "lattice"], default="ic",
                     help="initial state: device IC generator (default) or the jittered lattice")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-precision-form", action="store_true",
+                    help="skip the FP32 form's timing and error report")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 1 GiB device copies (scale for the PMC byte counters)")
     args = ap.parse_args()
@@ -262,17 +319,14 @@ def main():
     t_setup = time.perf_counter() - t_setup
     print("setup %.1f s (N=%d, box %.0f Mpc/h)" % (t_setup, N, boxsize), file=sys.stderr, flush=True)
 
-    box.run_cycles(sim, st, args.warmup)
-    box.synchronize()
-    if not args.no_timers:
-        box.timing_reset()
-        box.timing(True)
-    t0 = time.perf_counter()
-    box.run_cycles(sim, st, args.steps)
-    box.synchronize()
-    elapsed = time.perf_counter() - t0
-    kernels = box.timing_report() if not args.no_timers else {}
-    box.timing(False)
+    # the FP32 precision form starts from the same state (gev_copy_state)
+    box32 = st32 = None
+    if not args.no_precision_form:
+        box32 = gev.Box(N, Np, form=gev.FORM_FP32)
+        box32.copy_state_from(box)
+        st32 = gev.CycleState(st.a, st.tau, st.dtau, st.dtau_old, st.maxvel, st.T00hom, st.phi_k0, st.cycle, 0)
+
+    elapsed, kernels = timed_cycles(box, sim, st, args.warmup, args.steps, not args.no_timers)
     print("timed %d cycles in %.3f s; a = %.5f, T00hom = %.6f, maxvel = %.3e"
           % (args.steps, elapsed, st.a, st.T00hom, st.maxvel), file=sys.stderr, flush=True)
 
@@ -336,7 +390,11 @@ def main():
         "cpu_baseline": cpu,
         "kernels": breakdown,
         "ic_generator": ic_info,
+        "precision_form": (precision_form_report(gev, box, st, box32, st32, sim, args, ms_per_step)
+                           if box32 is not None else None),
     }
+    if box32 is not None:
+        box32.close()
     box.close()
     print(json.dumps(out))
 

@@ -103,11 +103,13 @@ static Sizes sizes_of(int N, int64_t cap) {
     return s;
 }
 
-static int64_t footprint(int N, int64_t cap) {
+static int64_t footprint(int N, int64_t cap, bool fp32) {
     Sizes s = sizes_of(N, cap);
+    const int64_t rb = fp32 ? 4 : 8, cb = 2 * rb;
     int64_t b = 0;
-    b += 12 * s.n3 * 8;             // phi, chi, source, Sij(6), Bi(3)
-    b += 11 * s.nk * 16;            // scalarFT, SijFT(6), BiFT(3), fft scratch
+    b += 12 * s.n3 * rb;            // phi, chi, source, Sij(6), Bi(3)
+    b += 11 * s.nk * cb;            // scalarFT, SijFT(6), BiFT(3), fft scratch
+    b += fp32 ? N * 8 : 0;          // float twiddles
     b += s.cap * (2 * (6 * 8 + 8) + 2 * 4);  // pos, q, id (two sets), cell, perm
     b += s.n3 * 4 + (s.n3 + 1) * 4 + s.nblk_scan * 4;
     b += s.nred * 8 + 8;
@@ -117,7 +119,7 @@ static int64_t footprint(int N, int64_t cap) {
 
 static void free_all(gev_ctx *c) {
     void *ptrs[] = {c->phi, c->chi, c->source, c->Sij, c->Bi, c->scalarFT, c->SijFT, c->BiFT,
-                    c->fft_tmp, c->twiddle, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
+                    c->fft_tmp, c->twiddle, c->twiddle_f, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
                     c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -153,17 +155,20 @@ using namespace gev;
 
 GEV_EXPORT int64_t gev_ctx_footprint_bytes(const gev_box_desc *d) {
     if (!d || d->ngrid < 2 || d->npart < 0) return -1;
-    return footprint(d->ngrid, d->npart);
+    if (d->form != GEV_FORM_FP64 && d->form != GEV_FORM_FP32) return -1;
+    return footprint(d->ngrid, d->npart, d->form == GEV_FORM_FP32);
 }
 
 GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     if (!out) return GEV_EINVAL;
     *out = nullptr;
     if (!d || d->npart < 0 || d->npart > 0xFFFFFFFFll) return GEV_EINVAL;
+    if (d->form != GEV_FORM_FP64 && d->form != GEV_FORM_FP32) return GEV_EINVAL;
     if (!fft_size_supported(d->ngrid)) return GEV_EINVAL;
     gev_ctx *c = new (std::nothrow) gev_ctx();
     if (!c) return GEV_ENOMEM;
     c->N = d->ngrid;
+    c->fp32 = d->form == GEV_FORM_FP32;
     c->P = kpitch(c->N);
     c->cap = d->npart;
     (void)hipGetDevice(&c->device);
@@ -185,16 +190,18 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
             ok = false;
         }
     };
-    A((void **)&c->phi, s.n3 * 8);
-    A((void **)&c->chi, s.n3 * 8);
-    A((void **)&c->source, s.n3 * 8);
-    A((void **)&c->Sij, 6 * s.n3 * 8);
-    A((void **)&c->Bi, 3 * s.n3 * 8);
-    A((void **)&c->scalarFT, s.nk * 16);
-    A((void **)&c->SijFT, 6 * s.nk * 16);
-    A((void **)&c->BiFT, 3 * s.nk * 16);
-    A((void **)&c->fft_tmp, s.nk * 16);
+    const int64_t rb = rbytes(c), cb = cbytes(c);
+    A((void **)&c->phi, s.n3 * rb);
+    A((void **)&c->chi, s.n3 * rb);
+    A((void **)&c->source, s.n3 * rb);
+    A((void **)&c->Sij, 6 * s.n3 * rb);
+    A((void **)&c->Bi, 3 * s.n3 * rb);
+    A((void **)&c->scalarFT, s.nk * cb);
+    A((void **)&c->SijFT, 6 * s.nk * cb);
+    A((void **)&c->BiFT, 3 * s.nk * cb);
+    A((void **)&c->fft_tmp, s.nk * cb);
     A((void **)&c->twiddle, c->N * 16);
+    if (c->fp32) A((void **)&c->twiddle_f, c->N * 8);
     A((void **)&c->gridk2, c->N * 8);
     A((void **)&c->kshift, c->N * 16);
     A((void **)&c->pos, 3 * s.cap * 8);
@@ -227,13 +234,18 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     rc = rc ? rc : check_hip(c, hipMemcpy(c->gridk2, g.data(), c->N * 8, hipMemcpyHostToDevice), "k tables");
     rc = rc ? rc : check_hip(c, hipMemcpy(c->kshift, ks.data(), c->N * 16, hipMemcpyHostToDevice), "k tables");
     rc = rc ? rc : check_hip(c, hipMemcpy(c->twiddle, tw.data(), c->N * 16, hipMemcpyHostToDevice), "twiddles");
+    if (c->fp32) {
+        std::vector<float2> twf(c->N);
+        for (int t = 0; t < c->N; t++) twf[t] = make_float2((float)tw[t].x, (float)tw[t].y);
+        rc = rc ? rc : check_hip(c, hipMemcpy(c->twiddle_f, twf.data(), c->N * 8, hipMemcpyHostToDevice), "twiddles");
+    }
     // every field starts at zero, like a fresh LATfield2 Field
     const size_t zero_sets[][2] = {
-        {(size_t)c->phi, (size_t)(s.n3 * 8)},        {(size_t)c->chi, (size_t)(s.n3 * 8)},
-        {(size_t)c->source, (size_t)(s.n3 * 8)},     {(size_t)c->Sij, (size_t)(6 * s.n3 * 8)},
-        {(size_t)c->Bi, (size_t)(3 * s.n3 * 8)},     {(size_t)c->scalarFT, (size_t)(s.nk * 16)},
-        {(size_t)c->SijFT, (size_t)(6 * s.nk * 16)}, {(size_t)c->BiFT, (size_t)(3 * s.nk * 16)},
-        {(size_t)c->fft_tmp, (size_t)(s.nk * 16)}};
+        {(size_t)c->phi, (size_t)(s.n3 * rb)},        {(size_t)c->chi, (size_t)(s.n3 * rb)},
+        {(size_t)c->source, (size_t)(s.n3 * rb)},     {(size_t)c->Sij, (size_t)(6 * s.n3 * rb)},
+        {(size_t)c->Bi, (size_t)(3 * s.n3 * rb)},     {(size_t)c->scalarFT, (size_t)(s.nk * cb)},
+        {(size_t)c->SijFT, (size_t)(6 * s.nk * cb)}, {(size_t)c->BiFT, (size_t)(3 * s.nk * cb)},
+        {(size_t)c->fft_tmp, (size_t)(s.nk * cb)}};
     for (auto &z : zero_sets)
         if (!rc) rc = launch_fill_zero(c, (void *)z[0], z[1]);
     if (!rc) rc = launch_refile(c);
@@ -364,11 +376,25 @@ GEV_EXPORT int gev_upload_field(gev_ctx *c, gev_field f, const double *host) {
     if (rc) return rc;
     rc = check_hip(c, hipStreamSynchronize(c->stream), "upload sync");
     if (rc) return rc;
+    const int H = c->N / 2 + 1;
+    if (c->fp32) {
+        // the host array is FP64 in both forms: rounded to float here
+        if (!cx) {
+            if ((int)f <= GEV_BI) c->zeroed[f] = false;
+            std::vector<float> t((size_t)nc * n3(c));
+            for (size_t i = 0; i < t.size(); i++) t[i] = (float)host[i];
+            return check_hip(c, hipMemcpy(p, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload field");
+        }
+        std::vector<float2> t((size_t)nc * c->N * c->N * H);
+        for (size_t i = 0; i < t.size(); i++) t[i] = make_float2((float)host[2 * i], (float)host[2 * i + 1]);
+        return check_hip(c, hipMemcpy2D(p, (size_t)c->P * 8, t.data(), (size_t)H * 8, (size_t)H * 8,
+                                        (size_t)nc * c->N * c->N, hipMemcpyHostToDevice),
+                         "upload k field");
+    }
     if (!cx) {
         if ((int)f <= GEV_BI) c->zeroed[f] = false;
         return check_hip(c, hipMemcpy(p, host, (size_t)nc * n3(c) * 8, hipMemcpyHostToDevice), "upload field");
     }
-    const int H = c->N / 2 + 1;
     return check_hip(c, hipMemcpy2D(p, (size_t)c->P * 16, host, (size_t)H * 16, (size_t)H * 16,
                                     (size_t)nc * c->N * c->N, hipMemcpyHostToDevice),
                      "upload k field");
@@ -383,12 +409,71 @@ GEV_EXPORT int gev_download_field(gev_ctx *c, gev_field f, double *host) {
     if (rc) return rc;
     rc = check_hip(c, hipStreamSynchronize(c->stream), "download sync");
     if (rc) return rc;
+    const int H = c->N / 2 + 1;
+    if (c->fp32) {
+        if (!cx) {
+            std::vector<float> t((size_t)nc * n3(c));
+            rc = check_hip(c, hipMemcpy(t.data(), p, t.size() * 4, hipMemcpyDeviceToHost), "download field");
+            for (size_t i = 0; !rc && i < t.size(); i++) host[i] = t[i];
+            return rc;
+        }
+        std::vector<float2> t((size_t)nc * c->N * c->N * H);
+        rc = check_hip(c, hipMemcpy2D(t.data(), (size_t)H * 8, p, (size_t)c->P * 8, (size_t)H * 8,
+                                      (size_t)nc * c->N * c->N, hipMemcpyDeviceToHost),
+                       "download k field");
+        for (size_t i = 0; !rc && i < t.size(); i++) {
+            host[2 * i] = t[i].x;
+            host[2 * i + 1] = t[i].y;
+        }
+        return rc;
+    }
     if (!cx)
         return check_hip(c, hipMemcpy(host, p, (size_t)nc * n3(c) * 8, hipMemcpyDeviceToHost), "download field");
-    const int H = c->N / 2 + 1;
     return check_hip(c, hipMemcpy2D(host, (size_t)H * 16, p, (size_t)c->P * 16, (size_t)H * 16,
                                     (size_t)nc * c->N * c->N, hipMemcpyDeviceToHost),
                      "download k field");
+}
+
+// The state of one context carried into another of the same grid size and
+// device, converting between the forms: particles (FP64 in both), the real
+// and k-space fields including the carried BiFT, the species mass, and the
+// filing (rebuilt from the positions).  The initial-condition generator needs
+// an FP64 context; this brings its result into an FP32 one.
+GEV_EXPORT int gev_copy_state(gev_ctx *src, gev_ctx *dst) {
+    if (!src || !dst) return GEV_EINVAL;
+    if (src == dst) return fail(dst, GEV_EINVAL, "source and destination are the same context");
+    if (src->N != dst->N) return fail(dst, GEV_EINVAL, "contexts differ in grid size");
+    if (src->device != dst->device) return fail(dst, GEV_EINVAL, "contexts live on different devices");
+    if (src->n > dst->cap) return fail(dst, GEV_EINVAL, "particle count exceeds capacity");
+    int rc = check_hip(dst, hipStreamSynchronize(src->stream), "copy state: source sync");
+    if (!rc) rc = check_hip(dst, hipStreamSynchronize(dst->stream), "copy state: destination sync");
+    if (rc) return rc;
+    const int64_t n = src->n;
+    for (int i = 0; i < 3 && !rc && n > 0; i++) {
+        rc = check_hip(dst, hipMemcpyAsync(dst->pos + (size_t)i * dst->cap, src->pos + (size_t)i * src->cap, n * 8,
+                                           hipMemcpyDeviceToDevice, dst->stream), "copy pos");
+        if (!rc)
+            rc = check_hip(dst, hipMemcpyAsync(dst->q + (size_t)i * dst->cap, src->q + (size_t)i * src->cap,
+                                               n * 8, hipMemcpyDeviceToDevice, dst->stream), "copy q");
+    }
+    if (!rc && n > 0)
+        rc = check_hip(dst, hipMemcpyAsync(dst->id, src->id, n * 8, hipMemcpyDeviceToDevice, dst->stream), "copy id");
+    const struct {
+        void *s, *d;
+        int64_t reals;
+    } f[] = {{src->phi, dst->phi, n3(src)},          {src->chi, dst->chi, n3(src)},
+             {src->source, dst->source, n3(src)},    {src->Sij, dst->Sij, 6 * n3(src)},
+             {src->Bi, dst->Bi, 3 * n3(src)},        {src->scalarFT, dst->scalarFT, 2 * nk(src)},
+             {src->SijFT, dst->SijFT, 12 * nk(src)}, {src->BiFT, dst->BiFT, 6 * nk(src)}};
+    for (const auto &e : f)
+        if (!rc) rc = launch_convert(dst, e.s, src->fp32, e.d, dst->fp32, e.reals);
+    if (rc) return rc;
+    for (int i = 0; i < 5; i++) dst->zeroed[i] = src->zeroed[i];
+    dst->n = n;
+    dst->pmass = src->pmass;
+    rc = launch_refile(dst);
+    if (!rc) rc = check_hip(dst, hipStreamSynchronize(dst->stream), "copy state sync");
+    return rc;
 }
 
 // ---------------------------------------------------------------- hot path
@@ -401,7 +486,7 @@ GEV_EXPORT int gev_projection_init(gev_ctx *c, gev_field f) {
     int rc = field_desc(c, f, &p, &nc, &cx);
     if (rc) return rc;
     if (cx) return fail(c, GEV_EINVAL, "projection_init needs a real field");
-    rc = launch_fill_zero(c, p, (size_t)nc * n3(c) * 8);
+    rc = launch_fill_zero(c, p, (size_t)nc * n3(c) * rbytes(c));
     if (!rc) c->zeroed[f] = true;
     return rc;
 }
@@ -562,6 +647,10 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, d
     const gev_cosmology *cs = &p->cosmo;
     double a = st->a;
     int rc;
+    // phi(k=0) and chi(k=0) of the z solves (FP64): element 0 of scalarFT in
+    // the FP64 form, FP64 slots of the reduction scratch in the FP32 form
+    double2 *const phik0 = c->fp32 ? reinterpret_cast<double2 *>(c->red_d + 2) : c->scalarFT;
+    double2 *const chik0 = c->fp32 ? reinterpret_cast<double2 *>(c->red_d + 4) : c->scalarFT;
 
     // projection_init(source), projection_init(Sij) + T00 and Tij deposits
     // (main.cpp:378-450): the fused deposit stores every cell, so the
@@ -593,18 +682,23 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, d
             if (!rc) rc = fft_forward_y(c, c->scalarFT, 1);
             if (!rc)
                 rc = launch_zsolve_poisson(c, c->scalarFT, c->fft_tmp, 1. / (dx * dx),
-                                           3. * hconf(a, fourpiG, cs) / st->dtau_old, nullptr, c->scalarFT);
+                                           3. * hconf(a, fourpiG, cs) / st->dtau_old, nullptr, phik0);
             if (!rc) rc = fft_backward_yx(c, c->fft_tmp, c->phi, 1);
             if (rc) return rc;
         }
     } else {
         rc = fft_forward_xy(c, c->source, c->scalarFT, 1);
-        if (!rc) rc = launch_zsolve_poisson(c, c->scalarFT, c->fft_tmp, fourpiG / a, 0., nullptr, c->scalarFT);
+        if (!rc) rc = launch_zsolve_poisson(c, c->scalarFT, c->fft_tmp, fourpiG / a, 0., nullptr, phik0);
         if (!rc) rc = fft_backward_yx(c, c->fft_tmp, c->phi, 1);
         if (rc) return rc;
     }
-    // phi(k=0) for the background record (main.cpp:533)
-    rc = check_hip(c, hipMemcpyAsync(c->red_d + 1, c->scalarFT, sizeof(double),
+    // phi(k=0) for the background record (main.cpp:533).  Without a solve in
+    // this cycle it is whatever scalarFT holds at k = 0 (widened in the FP32 form).
+    if (c->fp32 && p->gr_flag > 0 && !(st->dtau_old > 0.)) {
+        rc = launch_convert(c, c->scalarFT, true, phik0, false, 2);
+        if (rc) return rc;
+    }
+    rc = check_hip(c, hipMemcpyAsync(c->red_d + 1, phik0, sizeof(double),
                                      hipMemcpyDeviceToDevice, c->stream), "phi_k0");
     if (rc) return rc;
     if (rec) {
@@ -618,20 +712,22 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, d
     // after the solve: the z-inverted chi goes to its component 0 and B to
     // components 1-3; scalarFT keeps only chi(k=0); BiFT, the carried state,
     // is written in full.
-    double2 *const zout_chi = c->SijFT, *const zout_b = c->SijFT + nk(c);
+    double2 *const zout_chi = c->SijFT, *const zout_b = kcomp(c, c->SijFT, 1);
     rc = launch_x_prep_tensor(c, 2. * fourpiG * dx * dx / a);
     if (!rc) rc = fft_forward_y(c, c->SijFT, 6);
     if (rc) return rc;
     if (p->vector_flag == 1) {
-        rc = launch_zsolve_tensor(c, c->SijFT, zout_chi, 0., false, nullptr, c->scalarFT);
+        rc = launch_zsolve_tensor(c, c->SijFT, zout_chi, 0., false, nullptr, chik0);
         if (!rc) rc = fft_backward_yx(c, zout_chi, c->chi, 1);
         if (!rc) rc = fft_forward_xy(c, c->Bi, c->BiFT, 3);
         if (!rc) rc = launch_zsolve_vector(c, c->BiFT, zout_b, fourpiG * dx * dx, 0., c->BiFT);
     } else {
-        rc = launch_zsolve_tensor(c, c->SijFT, zout_chi, a * a * st->dtau_old, true, nullptr, c->scalarFT);
+        rc = launch_zsolve_tensor(c, c->SijFT, zout_chi, a * a * st->dtau_old, true, nullptr, chik0);
         if (!rc) rc = fft_backward_yx(c, zout_chi, c->chi, 1);
     }
     if (rc) return rc;
+    // scalarFT ends the cycle holding chi(k=0) in both forms
+    if (c->fp32 && (rc = launch_convert(c, chik0, false, c->scalarFT, true, 2))) return rc;
     if (p->gr_flag > 0) {
         rc = fft_backward_yx(c, zout_b, c->Bi, 3);
         if (rc) return rc;

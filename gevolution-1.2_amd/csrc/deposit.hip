@@ -277,24 +277,25 @@ __device__ __forceinline__ void add_source_cell(Acc &A, const ST &st, const Targ
     add_source<KIND>(A, P, T, DX, DY, DZ, mass_s);
 }
 
-// the target's sums, written once
-template <int KIND>
-__device__ __forceinline__ void store_target(const Acc &A, double *__restrict__ out_s, double *__restrict__ out_t,
-                                             uint32_t t, int64_t n3v) {
+// the target's sums, written once (rounded to F, the field element: double,
+// or float in the FP32 form; the sums themselves are FP64)
+template <int KIND, typename F>
+__device__ __forceinline__ void store_target(const Acc &A, F *__restrict__ out_s, F *__restrict__ out_t, uint32_t t,
+                                             int64_t n3v) {
     using K = Kind<KIND>;
-    if (K::T00) out_s[t] = A.s00;
+    if (K::T00) out_s[t] = (F)A.s00;
     if (K::TIJ) {
-        out_t[t] = A.d0;
-        out_t[3 * n3v + t] = A.d1;
-        out_t[5 * n3v + t] = A.d2;
-        out_t[n3v + t] = A.o01;
-        out_t[2 * n3v + t] = A.o02;
-        out_t[4 * n3v + t] = A.o12;
+        out_t[t] = (F)A.d0;
+        out_t[3 * n3v + t] = (F)A.d1;
+        out_t[5 * n3v + t] = (F)A.d2;
+        out_t[n3v + t] = (F)A.o01;
+        out_t[2 * n3v + t] = (F)A.o02;
+        out_t[4 * n3v + t] = (F)A.o12;
     }
     if (K::T0I) {
-        out_t[t] = A.v0;
-        out_t[n3v + t] = A.v1;
-        out_t[2 * n3v + t] = A.v2;
+        out_t[t] = (F)A.v0;
+        out_t[n3v + t] = (F)A.v1;
+        out_t[2 * n3v + t] = (F)A.v2;
     }
 }
 
@@ -322,11 +323,11 @@ __device__ __forceinline__ void add_records(Partial &P, const ST &st, const Targ
 
 // FULL: N is a multiple of the tile, so the tile and source-region extents
 // are compile-time constants (index arithmetic without integer division).
-template <int KIND, bool USE_PHI, bool ACCUM, bool POW2, bool FULL>
+template <int KIND, bool USE_PHI, bool ACCUM, bool POW2, bool FULL, typename F>
 __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const double *__restrict__ pos, const double *__restrict__ q, int64_t cap,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
-    const double *__restrict__ phi, double *__restrict__ out_s, double *__restrict__ out_t, int N,
+    const F *__restrict__ phi, F *__restrict__ out_s, F *__restrict__ out_t, int N,
     int ntx, int nty, int ntz, double a, double mass_s, double mass_t) {
     using K = Kind<KIND>;
     __shared__ Stage<KIND> st;
@@ -683,25 +684,24 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     if (active) store_target<KIND>(A, out_s, out_t, t, n3v);
 }
 
-template <int KIND, bool USE_PHI, bool ACCUM>
-void launch_t(gev_ctx *ctx, double *out_s, double *out_t, double a, double ms, double mt) {
+template <int KIND, bool USE_PHI, bool ACCUM, typename F>
+void launch_t(gev_ctx *ctx, F *out_s, F *out_t, double a, double ms, double mt) {
     const int N = ctx->N;
     const int ntx = (N + TX - 1) / TX, nty = (N + TY - 1) / TY, ntz = (N + TZ - 1) / TZ;
     const int64_t nblk = (int64_t)ntx * nty * ntz;
     const bool pow2 = (N & (N - 1)) == 0, full = N % TX == 0 && N % TY == 0 && N % TZ == 0;
 #define GEV_DEP_LAUNCH(P2, FL)                                                                              \
-    hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, P2, FL>), dim3((unsigned)nblk), dim3(TPB), 0, \
-                       ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, ctx->phi, out_s, out_t, \
-                       N, ntx, nty, ntz, a, ms, mt)
+    hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, P2, FL, F>), dim3((unsigned)nblk), dim3(TPB), 0, \
+                       ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, gev::fld<F>(ctx->phi), out_s, \
+                       out_t, N, ntx, nty, ntz, a, ms, mt)
     if (pow2 && full) GEV_DEP_LAUNCH(true, true);
     else if (full) GEV_DEP_LAUNCH(false, true);
     else GEV_DEP_LAUNCH(false, false);
 #undef GEV_DEP_LAUNCH
 }
 
-template <int KIND>
-void launch_k(gev_ctx *ctx, bool use_phi, bool accum, double *out_s, double *out_t, double a, double ms,
-              double mt) {
+template <int KIND, typename F>
+void launch_f(gev_ctx *ctx, bool use_phi, bool accum, F *out_s, F *out_t, double a, double ms, double mt) {
     if (use_phi) {
         if (accum) launch_t<KIND, true, true>(ctx, out_s, out_t, a, ms, mt);
         else launch_t<KIND, true, false>(ctx, out_s, out_t, a, ms, mt);
@@ -709,6 +709,13 @@ void launch_k(gev_ctx *ctx, bool use_phi, bool accum, double *out_s, double *out
         if (accum) launch_t<KIND, false, true>(ctx, out_s, out_t, a, ms, mt);
         else launch_t<KIND, false, false>(ctx, out_s, out_t, a, ms, mt);
     }
+}
+
+template <int KIND>
+void launch_k(gev_ctx *ctx, bool use_phi, bool accum, double *out_s, double *out_t, double a, double ms,
+              double mt) {
+    if (ctx->fp32) launch_f<KIND>(ctx, use_phi, accum, gev::fld<float>(out_s), gev::fld<float>(out_t), a, ms, mt);
+    else launch_f<KIND>(ctx, use_phi, accum, out_s, out_t, a, ms, mt);
 }
 
 }  // namespace

@@ -297,10 +297,14 @@ bool fft_size_supported(int N) {
     }
 }
 
+// the passes in the context's form: FP64, or FP32 (float fields, float2 k fields)
 static int forward(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp, int passes) {
     switch (ctx->N) {
-#define CASE(n) \
-    case n: return run_forward<double, n>(ctx, real, kfield, ncomp, ctx->twiddle, passes);
+#define CASE(n)                                                                                                  \
+    case n:                                                                                                      \
+        return ctx->fp32 ? run_forward<float, n>(ctx, fld<float>(real), cfld<float2>(kfield), ncomp, ctx->twiddle_f, \
+                                                 passes)                                                         \
+                         : run_forward<double, n>(ctx, real, kfield, ncomp, ctx->twiddle, passes);
         GEV_FFT_SIZES(CASE)
 #undef CASE
     default:
@@ -310,8 +314,11 @@ static int forward(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp,
 
 static int backward(gev_ctx *ctx, const double2 *kfield, double *real, int ncomp, bool from_y) {
     switch (ctx->N) {
-#define CASE(n) \
-    case n: return run_backward<double, n>(ctx, kfield, real, ncomp, ctx->fft_tmp, ctx->twiddle, from_y);
+#define CASE(n)                                                                                             \
+    case n:                                                                                                 \
+        return ctx->fp32 ? run_backward<float, n>(ctx, cfld<float2>(kfield), fld<float>(real), ncomp,       \
+                                                  cfld<float2>(ctx->fft_tmp), ctx->twiddle_f, from_y)       \
+                         : run_backward<double, n>(ctx, kfield, real, ncomp, ctx->fft_tmp, ctx->twiddle, from_y);
         GEV_FFT_SIZES(CASE)
 #undef CASE
     default:

@@ -68,30 +68,34 @@ template <int N, int KIND> constexpr int zs_threads() {
     return (t + 63) / 64 * 64;
 }
 
-struct ZArgs {
-    const double2 *in;   // NIN components, after the x and y passes
-    double2 *out;        // NOUT components, after the inverse z pass
-    double2 *bift;       // B state: read (tensor) / written (tensor, vector)
-    double2 *kstore;     // optional: solved k values of output 0 (full field)
-    double2 *k0;         // optional: output 0 at k = 0 (one element)
+// C: complex element of the k fields (double2, or float2 in the FP32 form).
+// The FFT stages run in C; the per-k solves run in FP64 either way (the k
+// tables, the denominators and the zero mode stay FP64).
+template <typename C> struct ZArgsT {
+    const C *in;         // NIN components, after the x and y passes
+    C *out;              // NOUT components, after the inverse z pass
+    C *bift;             // B state: read (tensor) / written (tensor, vector)
+    C *kstore;           // optional: solved k values of output 0 (full field)
+    double2 *k0;         // optional: output 0 at k = 0 (one element, FP64)
     const double *g;     // gridk2 per axis index
     const double2 *ks;   // kshift per axis index
-    const double2 *tw;   // exp(-2 pi i t / N), t < N
+    const C *tw;         // exp(-2 pi i t / N), t < N
     int64_t nkv;         // complexes per component
     int P, H;
     double c0, c1;       // Poisson: coeff/(-N^3), modif; tensor: a^2 dtau; vector: coeff, modif
 };
+using ZArgs = ZArgsT<double2>;
 
-template <int N, int KIND>
-__global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
+template <int N, int KIND, typename C>
+__global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgsT<C> A) {
     constexpr int W = zs_cols<N>();
     constexpr int NT = zs_threads<N, KIND>();
     constexpr int NIN = ZSolve<KIND>::NIN, NOUT = ZSolve<KIND>::NOUT;
     constexpr int PTS = (N * W + NT - 1) / NT;
     constexpr bool EXACT = PTS * NT == N * W;
     static_assert(NT % W == 0, "a thread keeps one column");
-    __shared__ double2 buf[N * W];
-    __shared__ double2 tw[N];
+    __shared__ C buf[N * W];
+    __shared__ C tw[N];
 
     // XCD-aware tile order: hardware workgroup ids go round-robin over the 8
     // XCDs, so tiles l and l+1 (which share 128-byte rows) are given to ids
@@ -111,34 +115,34 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
     for (int t = threadIdx.x; t < N; t += NT) tw[t] = A.tw[t];
 
     // column loads of input component c (element kz = idx / W of column w)
-    auto load_comp = [&](int c, double2 *v) {
+    auto load_comp = [&](int c, C *v) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
-            v[j] = c2(0., 0.);
+            v[j] = mk2<C>(0, 0);
             if (ok && (EXACT || idx < N * W)) v[j] = A.in[c * A.nkv + base + (int64_t)(idx / W) * es + w];
         }
     };
     // the carried B of the tensor solve, fetched while the last input transforms
-    double2 Bold[KIND == ZS_TENSOR ? 3 : 1][PTS];
-    auto load_bold = [&]() {
+    C Bold[KIND == ZS_TENSOR ? 3 : 1][PTS];
+    auto load_bold = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
             const int64_t at = base + (int64_t)(idx / W) * es + w;
 #pragma unroll
             for (int c = 0; c < (KIND == ZS_TENSOR ? 3 : 1); c++) {
-                Bold[c][j] = c2(0., 0.);
+                Bold[c][j] = mk2<C>(0, 0);
                 if (KIND == ZS_TENSOR && ok && (EXACT || idx < N * W)) Bold[c][j] = A.bift[c * A.nkv + at];
             }
         }
     };
 
-    double2 S[NIN][PTS];
+    C S[NIN][PTS];
     // input components in flight ahead of the one being transformed: the
     // next one, or the next two (ZS_AHEAD = 2, which reads B at the solve
     // instead of prefetching it)
-    double2 v[ZS_AHEAD][PTS];
+    C v[ZS_AHEAD][PTS];
 #pragma unroll
     for (int c = 0; c < ZS_AHEAD && c < NIN; c++) load_comp(c, v[c]);
 #pragma unroll
@@ -153,13 +157,13 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
         if (c + ZS_AHEAD < NIN) load_comp(c + ZS_AHEAD, v[c % ZS_AHEAD]);
         else if (c + 1 == NIN && KIND == ZS_TENSOR && zs_prefetch_b<N>()) load_bold();
         __syncthreads();
-        stages<double2, N, W, true, 1, NT>(buf, tw);
+        stages<C, N, W, true, 1, NT>(buf, tw);
         // each thread reads back only the points it wrote, so the next
         // component may overwrite them without another barrier
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
-            S[c][j] = (EXACT || idx < N * W) ? buf[idx] : c2(0., 0.);
+            S[c][j] = (EXACT || idx < N * W) ? buf[idx] : mk2<C>(0, 0);
         }
     }
 
@@ -179,50 +183,50 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
         const bool zero = kx == 0 && ky == 0 && kz == 0;
         const int64_t at = base + (int64_t)kz * es + w;
         if constexpr (KIND == ZS_POISSON) {
-            O[0][j] = poisson_point(S[0][j], zero, zero ? 0. : gx + gy + gz, A.c0, A.c1);
+            O[0][j] = poisson_point(d2(S[0][j]), zero, zero ? 0. : gx + gy + gz, A.c0, A.c1);
         } else if constexpr (KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR) {
             double2 Sp[6];
 #pragma unroll
-            for (int c = 0; c < 6; c++) Sp[c] = S[c][j];
+            for (int c = 0; c < 6; c++) Sp[c] = d2(S[c][j]);
             O[0][j] = zero ? c2(0., 0.) : project_scalar(Sp, gx, gy, gz, kxs, kys, kzs, N);
             if constexpr (KIND == ZS_TENSOR) {
                 double2 B[3] = {c2(0., 0.), c2(0., 0.), c2(0., 0.)};
                 if (live) {
                     if (!zero) {
                         if constexpr (zs_prefetch_b<N>()) {
-                            B[0] = Bold[0][j];
-                            B[1] = Bold[KIND == ZS_TENSOR ? 1 : 0][j];
-                            B[2] = Bold[KIND == ZS_TENSOR ? 2 : 0][j];
+                            B[0] = d2(Bold[0][j]);
+                            B[1] = d2(Bold[KIND == ZS_TENSOR ? 1 : 0][j]);
+                            B[2] = d2(Bold[KIND == ZS_TENSOR ? 2 : 0][j]);
                         } else {
-                            B[0] = A.bift[at];
-                            B[1] = A.bift[A.nkv + at];
-                            B[2] = A.bift[2 * A.nkv + at];
+                            B[0] = d2(A.bift[at]);
+                            B[1] = d2(A.bift[A.nkv + at]);
+                            B[2] = d2(A.bift[2 * A.nkv + at]);
                         }
                         evolve_vector(Sp, B, gx, gy, gz, kxs, kys, kzs, A.c0);
                     }
-                    A.bift[at] = B[0];
-                    A.bift[A.nkv + at] = B[1];
-                    A.bift[2 * A.nkv + at] = B[2];
+                    A.bift[at] = from_d2<C>(B[0]);
+                    A.bift[A.nkv + at] = from_d2<C>(B[1]);
+                    A.bift[2 * A.nkv + at] = from_d2<C>(B[2]);
                 }
 #pragma unroll
                 for (int c = 0; c < 3; c++) O[1 + c][j] = B[c];
             }
         } else {
-            double2 Sv[3] = {S[0][j], S[1][j], S[2][j]};
+            double2 Sv[3] = {d2(S[0][j]), d2(S[1][j]), d2(S[2][j])};
             if (zero) {
                 Sv[0] = Sv[1] = Sv[2] = c2(0., 0.);
             } else {
                 project_vector_point(Sv, kxs, kys, kzs, gx + gy + gz, A.c0, A.c1);
             }
             if (live && A.bift) {
-                A.bift[at] = Sv[0];
-                A.bift[A.nkv + at] = Sv[1];
-                A.bift[2 * A.nkv + at] = Sv[2];
+                A.bift[at] = from_d2<C>(Sv[0]);
+                A.bift[A.nkv + at] = from_d2<C>(Sv[1]);
+                A.bift[2 * A.nkv + at] = from_d2<C>(Sv[2]);
             }
 #pragma unroll
             for (int c = 0; c < 3; c++) O[c][j] = Sv[c];
         }
-        if (live && A.kstore) A.kstore[at] = O[0][j];
+        if (live && A.kstore) A.kstore[at] = from_d2<C>(O[0][j]);
         if (zero && A.k0) A.k0[0] = O[0][j];
     }
 
@@ -232,10 +236,10 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
-            if (EXACT || idx < N * W) buf[idx] = cconj(O[o][j]);
+            if (EXACT || idx < N * W) buf[idx] = from_d2<C>(cconj(O[o][j]));
         }
         __syncthreads();
-        stages<double2, N, W, true, 1, NT>(buf, tw);
+        stages<C, N, W, true, 1, NT>(buf, tw);
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
@@ -251,15 +255,16 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgs A) {
 // deposited source (1 or 6 components) and phi (and chi) are read, the
 // prepared values never reach HBM.  Same half-length transform and
 // post-processing as k_r2c_x.
-struct XArgs {
-    const double *phi, *chi;  // chi: scalar form only
-    const double *src;        // NC deposited components, N^3 each
-    double2 *out;             // NC k fields after the x pass
-    const double2 *tw;
+template <typename F, typename C> struct XArgsT {
+    const F *phi, *chi;       // chi: scalar form only
+    const F *src;             // NC deposited components, N^3 each
+    C *out;                   // NC k fields after the x pass
+    const C *tw;
     int P;
     int64_t nrows, n3v, nkv;
     double c0, c1, c2, c3;  // tensor: coeff; scalar: bg, coeff, coeff2, coeff3
 };
+using XArgs = XArgsT<double, double2>;
 
 // Rows per workgroup: the NC prepared components of a thread's points are
 // held in registers until each is transformed, so the 6-component form
@@ -273,25 +278,26 @@ template <int N, int NC> constexpr int xprep_lines() {
     return l;
 }
 
-template <int N, int NC>
-__global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
+template <int N, int NC, typename F, typename C>
+__global__ __launch_bounds__(256) void k_r2c_x_prep(XArgsT<F, C> A) {
+    using R = decltype(C::x);
     constexpr int TPB = 256;
     constexpr int M = N / 2;
     constexpr int LINES = xprep_lines<N, NC>();
     constexpr int PER = (LINES * M + TPB - 1) / TPB;
     constexpr bool EXACT = PER * TPB == LINES * M;
-    __shared__ double2 buf[LINES * xstride<M>()];
-    __shared__ double2 tw[M];
+    __shared__ C buf[LINES * xstride<M>()];
+    __shared__ C tw[M];
     const int64_t row0 = (int64_t)blockIdx.x * LINES;
     for (int t = threadIdx.x; t < M; t += TPB) tw[t] = A.tw[2 * t];
-    double2 v[NC][PER];
+    C v[NC][PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int idx = threadIdx.x + k * TPB;
         const int line = idx / M, n = idx % M;
         const int64_t row = row0 + line;
 #pragma unroll
-        for (int c = 0; c < NC; c++) v[c][k] = c2(0., 0.);
+        for (int c = 0; c < NC; c++) v[c][k] = mk2<C>(0, 0);
         if (!((EXACT || idx < LINES * M) && row < A.nrows)) continue;
         const int y = (int)(row % N), z = (int)(row / N);
 #pragma unroll
@@ -306,13 +312,13 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
                 prep_tensor(A.phi, T, c, i, N, A.c0, S);
 #pragma unroll
                 for (int q = 0; q < 6; q++) {
-                    if (h == 0) v[q][k].x = S[q];
-                    else v[q][k].y = S[q];
+                    if (h == 0) v[q][k].x = (R)S[q];
+                    else v[q][k].y = (R)S[q];
                 }
             } else {
-                const double r = prep_scalar(A.phi, A.chi[i], A.src[i], c, i, N, A.c0, A.c1, A.c2, A.c3);
-                if (h == 0) v[0][k].x = r;
-                else v[0][k].y = r;
+                const double r = prep_scalar(A.phi, (double)A.chi[i], (double)A.src[i], c, i, N, A.c0, A.c1, A.c2, A.c3);
+                if (h == 0) v[0][k].x = (R)r;
+                else v[0][k].y = (R)r;
             }
         }
     }
@@ -324,37 +330,58 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgs A) {
             if (EXACT || idx < LINES * M) buf[eaddr<M, LINES, false>(idx / M, idx % M)] = v[q][k];
         }
         __syncthreads();
-        stages<double2, M, LINES, false, 1>(buf, tw);
+        stages<C, M, LINES, false, 1>(buf, tw);
         for (int idx = threadIdx.x; idx < LINES * (M + 1); idx += TPB) {
             const int line = idx / (M + 1), k = idx % (M + 1);
             const int64_t row = row0 + line;
             if (row >= A.nrows) continue;
-            const double2 Ak = buf[eaddr<M, LINES, false>(line, k == M ? 0 : k)];
-            const double2 Bk = cconj(buf[eaddr<M, LINES, false>(line, k == 0 ? 0 : M - k)]);
-            const double2 E = c2((Ak.x + Bk.x) * 0.5, (Ak.y + Bk.y) * 0.5);
-            const double2 D = c2((Ak.x - Bk.x) * 0.5, (Ak.y - Bk.y) * 0.5);
-            const double2 O = mul_mi(D);
+            const C Ak = buf[eaddr<M, LINES, false>(line, k == M ? 0 : k)];
+            const C Bk = cconj(buf[eaddr<M, LINES, false>(line, k == 0 ? 0 : M - k)]);
+            const C E = mk2<C>((Ak.x + Bk.x) * (R)0.5, (Ak.y + Bk.y) * (R)0.5);
+            const C D = mk2<C>((Ak.x - Bk.x) * (R)0.5, (Ak.y - Bk.y) * (R)0.5);
+            const C O = mul_mi(D);
             A.out[q * A.nkv + row * A.P + k] = cadd(E, cmul(A.tw[k], O));
         }
         __syncthreads();  // the next component overwrites buf
     }
 }
 
-template <int N, int NC> int launch_x(gev_ctx *c, const XArgs &A) {
+template <int N, int NC, typename F, typename C> int launch_x(gev_ctx *c, const XArgsT<F, C> &A) {
     constexpr int LINES = xprep_lines<N, NC>();
-    hipLaunchKernelGGL((k_r2c_x_prep<N, NC>), dim3(gev::grid_for(A.nrows, LINES)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL((k_r2c_x_prep<N, NC, F, C>), dim3(gev::grid_for(A.nrows, LINES)), dim3(256), 0, c->stream,
+                       A);
     return gev::check_hip(c, hipGetLastError(), "fused x pass launch");
 }
 
+// the context's fields as F / C (double / double2, or float / float2 in the
+// FP32 form)
+template <typename F, typename C> XArgsT<F, C> xargs_as(const XArgs &A) {
+    XArgsT<F, C> B{};
+    B.phi = gev::fld<F>(const_cast<double *>(A.phi));
+    B.chi = gev::fld<F>(const_cast<double *>(A.chi));
+    B.src = gev::fld<F>(const_cast<double *>(A.src));
+    B.out = gev::cfld<C>(A.out);
+    B.tw = gev::cfld<C>(const_cast<double2 *>(A.tw));
+    B.P = A.P;
+    B.nrows = A.nrows;
+    B.n3v = A.n3v;
+    B.nkv = A.nkv;
+    B.c0 = A.c0;
+    B.c1 = A.c1;
+    B.c2 = A.c2;
+    B.c3 = A.c3;
+    return B;
+}
+
 template <int NC> int launch_x_nc(gev_ctx *c, XArgs A) {
-    A.tw = c->twiddle;
+    A.tw = c->fp32 ? reinterpret_cast<const double2 *>(c->twiddle_f) : c->twiddle;
     A.P = c->P;
     A.nrows = (int64_t)c->N * c->N;
     A.n3v = gev::n3(c);
     A.nkv = gev::nk(c);
     switch (c->N) {
 #define CASE(n) \
-    case n: return launch_x<n, NC>(c, A);
+    case n: return c->fp32 ? launch_x<n, NC>(c, xargs_as<float, float2>(A)) : launch_x<n, NC>(c, A);
         GEV_FFT_SIZES(CASE)
 #undef CASE
     default:
@@ -364,23 +391,42 @@ template <int NC> int launch_x_nc(gev_ctx *c, XArgs A) {
 
 // ---------------------------------------------------------------- last pass
 
-template <int N, int KIND> int launch_n(gev_ctx *c, const ZArgs &A) {
+template <int N, int KIND, typename C> int launch_n(gev_ctx *c, const ZArgsT<C> &A) {
     constexpr int W = zs_cols<N>();
     const int H = N / 2 + 1;
-    hipLaunchKernelGGL((k_zsolve<N, KIND>), dim3((H + W - 1) / W, N), dim3(zs_threads<N, KIND>()), 0, c->stream, A);
+    hipLaunchKernelGGL((k_zsolve<N, KIND, C>), dim3((H + W - 1) / W, N), dim3(zs_threads<N, KIND>()), 0, c->stream,
+                       A);
     return gev::check_hip(c, hipGetLastError(), "fused z solve launch");
+}
+
+template <typename C> ZArgsT<C> zargs_as(const ZArgs &A) {
+    ZArgsT<C> B{};
+    B.in = gev::cfld<C>(const_cast<double2 *>(A.in));
+    B.out = gev::cfld<C>(A.out);
+    B.bift = gev::cfld<C>(A.bift);
+    B.kstore = gev::cfld<C>(A.kstore);
+    B.k0 = A.k0;
+    B.g = A.g;
+    B.ks = A.ks;
+    B.tw = gev::cfld<C>(const_cast<double2 *>(A.tw));
+    B.nkv = A.nkv;
+    B.P = A.P;
+    B.H = A.H;
+    B.c0 = A.c0;
+    B.c1 = A.c1;
+    return B;
 }
 
 template <int KIND> int launch_kind(gev_ctx *c, ZArgs A) {
     A.g = c->gridk2;
     A.ks = c->kshift;
-    A.tw = c->twiddle;
+    A.tw = c->fp32 ? reinterpret_cast<const double2 *>(c->twiddle_f) : c->twiddle;
     A.nkv = gev::nk(c);
     A.P = c->P;
     A.H = c->N / 2 + 1;
     switch (c->N) {
 #define CASE(n) \
-    case n: return launch_n<n, KIND>(c, A);
+    case n: return c->fp32 ? launch_n<n, KIND>(c, zargs_as<float2>(A)) : launch_n<n, KIND>(c, A);
         GEV_FFT_SIZES(CASE)
 #undef CASE
     default:

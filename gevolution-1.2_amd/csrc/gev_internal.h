@@ -44,12 +44,17 @@ struct gev_ctx {
     bool own_stream = false;
     int device = 0;
 
-    // real fields, N^3 doubles per component
+    // FP32 form: the fields below hold float / float2 elements (the pointers
+    // keep their FP64 type as handles; launchers cast with fld<F>()); the
+    // particles, the k-space tables and the background stay FP64.
+    bool fp32 = false;
+    // real fields, N^3 elements per component
     double *phi = nullptr, *chi = nullptr, *source = nullptr, *Sij = nullptr, *Bi = nullptr;
     // complex fields, P*N*N complexes per component
     double2 *scalarFT = nullptr, *SijFT = nullptr, *BiFT = nullptr, *fft_tmp = nullptr;
-    // FFT twiddles exp(-2 pi i t/N), t < N; k-space tables
+    // FFT twiddles exp(-2 pi i t/N), t < N (float copy for the FP32 form); k-space tables
     double2 *twiddle = nullptr;
+    float2 *twiddle_f = nullptr;
     double *gridk2 = nullptr;   // (2N sin(pi n/N))^2
     double2 *kshift = nullptr;  // 2N sin(pi n/N) exp(-i pi n/N)
 
@@ -104,7 +109,20 @@ struct ScopedTimer {
 };
 
 inline int64_t n3(const gev_ctx *c) { return (int64_t)c->N * c->N * c->N; }
+// element bytes of a real field and of a complex field in the context's form
+inline int64_t rbytes(const gev_ctx *c) { return c->fp32 ? 4 : 8; }
+inline int64_t cbytes(const gev_ctx *c) { return c->fp32 ? 8 : 16; }
+// a field handle as the element type of the form in use
+template <typename F> inline F *fld(double *p) { return reinterpret_cast<F *>(p); }
+template <typename F> inline const F *fld(const double *p) { return reinterpret_cast<const F *>(p); }
+template <typename C> inline C *cfld(double2 *p) { return reinterpret_cast<C *>(p); }
+template <typename C> inline const C *cfld(const double2 *p) { return reinterpret_cast<const C *>(p); }
 inline int64_t nk(const gev_ctx *c) { return (int64_t)c->P * c->N * c->N; }
+// component k of a complex field handle (components are nk elements apart in
+// the form's element size)
+inline double2 *kcomp(const gev_ctx *c, double2 *p, int k) {
+    return reinterpret_cast<double2 *>(reinterpret_cast<char *>(p) + (size_t)k * nk(c) * cbytes(c));
+}
 
 inline unsigned grid_for(int64_t work, int block) {
     return (unsigned)((work + block - 1) / block);
@@ -167,6 +185,7 @@ int launch_prepare_scalar(gev_ctx *ctx, double bg, double c1, double c2, double 
 int launch_prepare_tensor(gev_ctx *ctx, double coeff);
 int launch_field_sum(gev_ctx *ctx, const double *f, int64_t count, double *out_host);
 int launch_fill_zero(gev_ctx *ctx, void *ptr, size_t bytes);
+int launch_convert(gev_ctx *ctx, const void *src, bool src32, void *dst, bool dst32, int64_t n);
 
 enum DepositKind { DEP_T00 = 1, DEP_TIJ = 2, DEP_T00_TIJ = 3, DEP_T0I = 4, DEP_CIC = 5 };
 int launch_deposit(gev_ctx *ctx, int kind, double a, int use_phi, double coeff);

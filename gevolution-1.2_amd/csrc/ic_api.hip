@@ -54,7 +54,16 @@ std::vector<double> spline_coeffs(const double *x, const double *y, int n) {
     return c;
 }
 
+// The initial-condition generator evaluates its fields in FP64: in the FP32
+// form it is run on an FP64 context whose state gev_copy_state then carries over.
+int fp64_only(gev_ctx *c) {
+    return c->fp32 ? fail(c, GEV_EINVAL,
+                          "the initial-condition generator needs an FP64 context (generate there, then gev_copy_state)")
+                   : 0;
+}
+
 int real_field(gev_ctx *c, gev_field f, double **out) {
+    if (int rc = fp64_only(c)) return rc;
     switch (f) {
     case GEV_PHI: *out = c->phi; return 0;
     case GEV_CHI: *out = c->chi; return 0;
@@ -329,6 +338,7 @@ GEV_EXPORT int gev_generate_displacement_field(gev_ctx *c, double coeff, int32_t
                                                const double *y, uint32_t seed, int32_t ksphere,
                                                int32_t deconvolve_f, int32_t ignorekernel) {
     if (!c) return GEV_EINVAL;
+    if (int rc = fp64_only(c)) return rc;
     if (nknots < 2 || !x || !y) return fail(c, GEV_EINVAL, "spline needs two or more knots");
     for (int i = 1; i < nknots; i++)
         if (!(x[i] > x[i - 1])) return fail(c, GEV_EINVAL, "spline knots must increase");
@@ -400,6 +410,7 @@ GEV_EXPORT int gev_generate_ic_basic(gev_ctx *c, const gev_sim_params *p, const 
                                      const double *d_tot, const double *t_tot, const double *d_cdm,
                                      const double *t_cdm, double *maxvel, double *max_displacement) {
     if (!c) return GEV_EINVAL;
+    if (int rc = fp64_only(c)) return rc;
     if (!p || !ic || !pcldata || numpcl < 1 || ic->numtile < 1 || ntk < 2 || !k || !d_tot || !t_tot ||
         !d_cdm || !t_cdm)
         return fail(c, GEV_EINVAL, "bad initial-condition arguments");

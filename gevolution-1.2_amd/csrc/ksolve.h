@@ -20,6 +20,16 @@ __device__ __forceinline__ double2 cmulr(double2 a, double s) { return c2(a.x * 
 __device__ __forceinline__ double2 cdivr(double2 a, double s) { return c2(a.x / s, a.y / s); }
 __device__ __forceinline__ double2 cconj(double2 a) { return c2(a.x, -a.y); }
 
+// a k value in storage (double2, or float2 in the FP32 form) to FP64 and back:
+// the solves below always evaluate in FP64
+__device__ __forceinline__ double2 d2(double2 v) { return v; }
+__device__ __forceinline__ double2 d2(float2 v) { return make_double2(v.x, v.y); }
+template <typename C> __device__ __forceinline__ C from_d2(double2 v);
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 v) { return v; }
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 v) {
+    return make_float2((float)v.x, (float)v.y);
+}
+
 // solveModifiedPoissonFT body, gevolution.hpp:501-535 (coeff already divided by -N^3)
 __device__ __forceinline__ double2 poisson_point(double2 s, bool zero, double k2, double coeff,
                                                  double modif) {

@@ -29,22 +29,26 @@ __device__ __forceinline__ KIdx kindex(int64_t i, int P, int N) {
     return k;
 }
 
+// C: the k-field element (double2, or float2 in the FP32 form); the solves
+// evaluate in FP64 either way.
+
 // solveModifiedPoissonFT, gevolution.hpp:501-535 (coeff already divided by -N^3)
-__global__ __launch_bounds__(TPB) void k_poisson(double2 *__restrict__ ft, const double *__restrict__ g,
-                                                 int P, int N, int64_t nkv, double coeff, double modif) {
+template <typename C>
+__global__ __launch_bounds__(TPB) void k_poisson(C *__restrict__ ft, const double *__restrict__ g, int P, int N,
+                                                 int64_t nkv, double coeff, double modif) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= nkv) return;
     KIdx k = kindex(i, P, N);
     if (!k.valid) return;
     const bool zero = k.kx == 0 && k.ky == 0 && k.kz == 0;
-    ft[i] = poisson_point(ft[i], zero, zero ? 0. : g[k.kx] + g[k.ky] + g[k.kz], coeff, modif);
+    ft[i] = from_d2<C>(poisson_point(d2(ft[i]), zero, zero ? 0. : g[k.kx] + g[k.ky] + g[k.kz], coeff, modif));
 }
 
 // MODE bit 1: projectFTscalar -> chi; bit 2: evolveFTvector -> BiFT
-template <int MODE>
-__global__ __launch_bounds__(TPB) void k_sij_solve(const double2 *__restrict__ SijFT,
-                                                   double2 *__restrict__ chiFT,
-                                                   double2 *__restrict__ BiFT,
+template <int MODE, typename C>
+__global__ __launch_bounds__(TPB) void k_sij_solve(const C *__restrict__ SijFT,
+                                                   C *__restrict__ chiFT,
+                                                   C *__restrict__ BiFT,
                                                    const double *__restrict__ g,
                                                    const double2 *__restrict__ ks, int P, int N,
                                                    int64_t nkv, int add, double a2dtau) {
@@ -55,30 +59,31 @@ __global__ __launch_bounds__(TPB) void k_sij_solve(const double2 *__restrict__ S
     const bool zero = k.kx == 0 && k.ky == 0 && k.kz == 0;
     double2 S[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) S[c] = SijFT[c * nkv + i];
+    for (int c = 0; c < 6; c++) S[c] = d2(SijFT[c * nkv + i]);
     const double g0 = g[k.kx], g1 = g[k.ky], g2 = g[k.kz];
     const double2 k0 = ks[k.kx], k1 = ks[k.ky], k2 = ks[k.kz];
     if (MODE & 1) {
         if (zero) {
-            chiFT[i] = c2(0., 0.);
+            chiFT[i] = from_d2<C>(c2(0., 0.));
         } else {
             double2 r = project_scalar(S, g0, g1, g2, k0, k1, k2, N);
-            chiFT[i] = add ? cadd(chiFT[i], r) : r;
+            chiFT[i] = from_d2<C>(add ? cadd(d2(chiFT[i]), r) : r);
         }
     }
     if (MODE & 2) {
         if (zero) {
-            for (int c = 0; c < 3; c++) BiFT[c * nkv + i] = c2(0., 0.);
+            for (int c = 0; c < 3; c++) BiFT[c * nkv + i] = from_d2<C>(c2(0., 0.));
         } else {
-            double2 B[3] = {BiFT[i], BiFT[nkv + i], BiFT[2 * nkv + i]};
+            double2 B[3] = {d2(BiFT[i]), d2(BiFT[nkv + i]), d2(BiFT[2 * nkv + i])};
             evolve_vector(S, B, g0, g1, g2, k0, k1, k2, a2dtau);
-            for (int c = 0; c < 3; c++) BiFT[c * nkv + i] = B[c];
+            for (int c = 0; c < 3; c++) BiFT[c * nkv + i] = from_d2<C>(B[c]);
         }
     }
 }
 
 // projectFTvector, gevolution.hpp:350-392 (in place on BiFT)
-__global__ __launch_bounds__(TPB) void k_project_vector(double2 *__restrict__ B,
+template <typename C>
+__global__ __launch_bounds__(TPB) void k_project_vector(C *__restrict__ B,
                                                         const double *__restrict__ g,
                                                         const double2 *__restrict__ ks, int P,
                                                         int N, int64_t nkv, double coeff,
@@ -88,14 +93,26 @@ __global__ __launch_bounds__(TPB) void k_project_vector(double2 *__restrict__ B,
     KIdx k = kindex(i, P, N);
     if (!k.valid) return;
     if (k.kx == 0 && k.ky == 0 && k.kz == 0) {
-        for (int c = 0; c < 3; c++) B[c * nkv + i] = c2(0., 0.);
+        for (int c = 0; c < 3; c++) B[c * nkv + i] = from_d2<C>(c2(0., 0.));
         return;
     }
-    double2 S[3] = {B[i], B[nkv + i], B[2 * nkv + i]};
+    double2 S[3] = {d2(B[i]), d2(B[nkv + i]), d2(B[2 * nkv + i])};
     project_vector_point(S, ks[k.kx], ks[k.ky], ks[k.kz], g[k.kx] + g[k.ky] + g[k.kz], coeff, modif);
-    B[i] = S[0];
-    B[nkv + i] = S[1];
-    B[2 * nkv + i] = S[2];
+    B[i] = from_d2<C>(S[0]);
+    B[nkv + i] = from_d2<C>(S[1]);
+    B[2 * nkv + i] = from_d2<C>(S[2]);
+}
+
+template <int MODE> void sij_solve(gev_ctx *ctx, int add, double a2dtau) {
+    if (ctx->fp32)
+        hipLaunchKernelGGL((k_sij_solve<MODE, float2>), dim3(gev::grid_for(gev::nk(ctx), TPB)), dim3(TPB), 0,
+                           ctx->stream, gev::cfld<float2>(ctx->SijFT), gev::cfld<float2>(ctx->scalarFT),
+                           gev::cfld<float2>(ctx->BiFT), ctx->gridk2, ctx->kshift, ctx->P, ctx->N, gev::nk(ctx), add,
+                           a2dtau);
+    else
+        hipLaunchKernelGGL((k_sij_solve<MODE, double2>), dim3(gev::grid_for(gev::nk(ctx), TPB)), dim3(TPB), 0,
+                           ctx->stream, ctx->SijFT, ctx->scalarFT, ctx->BiFT, ctx->gridk2, ctx->kshift, ctx->P,
+                           ctx->N, gev::nk(ctx), add, a2dtau);
 }
 
 }  // namespace
@@ -106,39 +123,42 @@ int launch_poisson(gev_ctx *ctx, double coeff, double modif) {
     const int N = ctx->N;
     coeff /= -((long)N * (long)N * (long)N);  // gevolution.hpp:511
     ScopedTimer t(ctx, "solve_modified_poisson_ft");
-    hipLaunchKernelGGL(k_poisson, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
-                       ctx->scalarFT, ctx->gridk2, ctx->P, N, nk(ctx), coeff, modif);
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_poisson<float2>, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           cfld<float2>(ctx->scalarFT), ctx->gridk2, ctx->P, N, nk(ctx), coeff, modif);
+    else
+        hipLaunchKernelGGL(k_poisson<double2>, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           ctx->scalarFT, ctx->gridk2, ctx->P, N, nk(ctx), coeff, modif);
     return check_hip(ctx, hipGetLastError(), "poisson launch");
 }
 
 int launch_project_scalar(gev_ctx *ctx, int add) {
     ScopedTimer t(ctx, "project_ft_scalar");
-    hipLaunchKernelGGL(k_sij_solve<1>, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
-                       ctx->SijFT, ctx->scalarFT, ctx->BiFT, ctx->gridk2, ctx->kshift, ctx->P,
-                       ctx->N, nk(ctx), add, 0.0);
+    sij_solve<1>(ctx, add, 0.0);
     return check_hip(ctx, hipGetLastError(), "project scalar launch");
 }
 
 int launch_evolve_vector(gev_ctx *ctx, double a2dtau) {
     ScopedTimer t(ctx, "evolve_ft_vector");
-    hipLaunchKernelGGL(k_sij_solve<2>, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
-                       ctx->SijFT, ctx->scalarFT, ctx->BiFT, ctx->gridk2, ctx->kshift, ctx->P,
-                       ctx->N, nk(ctx), 0, a2dtau);
+    sij_solve<2>(ctx, 0, a2dtau);
     return check_hip(ctx, hipGetLastError(), "evolve vector launch");
 }
 
 int launch_project_scalar_evolve_vector(gev_ctx *ctx, double a2dtau) {
     ScopedTimer t(ctx, "project_scalar_evolve_vector");
-    hipLaunchKernelGGL(k_sij_solve<3>, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
-                       ctx->SijFT, ctx->scalarFT, ctx->BiFT, ctx->gridk2, ctx->kshift, ctx->P,
-                       ctx->N, nk(ctx), 0, a2dtau);
+    sij_solve<3>(ctx, 0, a2dtau);
     return check_hip(ctx, hipGetLastError(), "fused k solve launch");
 }
 
 int launch_project_vector(gev_ctx *ctx, double coeff, double modif) {
     ScopedTimer t(ctx, "project_ft_vector");
-    hipLaunchKernelGGL(k_project_vector, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
-                       ctx->BiFT, ctx->gridk2, ctx->kshift, ctx->P, ctx->N, nk(ctx), coeff, modif);
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_project_vector<float2>, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           cfld<float2>(ctx->BiFT), ctx->gridk2, ctx->kshift, ctx->P, ctx->N, nk(ctx), coeff,
+                           modif);
+    else
+        hipLaunchKernelGGL(k_project_vector<double2>, dim3(grid_for(nk(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           ctx->BiFT, ctx->gridk2, ctx->kshift, ctx->P, ctx->N, nk(ctx), coeff, modif);
     return check_hip(ctx, hipGetLastError(), "project vector launch");
 }
 

@@ -216,22 +216,24 @@ template <int LO> struct Region {
 };
 
 // one field around the particle's cell: (dx, dy, dz) within the region
-template <int LO> struct FieldView {
-    const double *f;  // LDS image of the field over its staged region
-    int base;         // region index of the particle's cell
+// (F: the field element, double, or float in the FP32 form; values are read
+// as double)
+template <int LO, typename F> struct FieldView {
+    const F *f;  // LDS image of the field over its staged region
+    int base;    // region index of the particle's cell
     __device__ __forceinline__ double operator()(int dx, int dy, int dz) const {
-        return f[base + dx + Region<LO>::SX * (dy + Region<LO>::SY * dz)];
+        return (double)f[base + dx + Region<LO>::SX * (dy + Region<LO>::SY * dz)];
     }
 };
-template <int LO> __device__ __forceinline__ FieldView<LO> view(const double *f, const int c[3], int x0, int y0,
-                                                                 int z0) {
+template <int LO, typename F>
+__device__ __forceinline__ FieldView<LO, F> view(const F *f, const int c[3], int x0, int y0, int z0) {
     using R = Region<LO>;
-    return FieldView<LO>{f, (c[0] - x0 + R::OX) + R::SX * ((c[1] - y0 + R::OY) + R::SY * (c[2] - z0 + R::OZ))};
+    return FieldView<LO, F>{f, (c[0] - x0 + R::OX) + R::SX * ((c[1] - y0 + R::OY) + R::SY * (c[2] - z0 + R::OZ))};
 }
-using ViewS = FieldView<-1>;  // phi, chi
-using View0 = FieldView<0>;
-using View1 = FieldView<1>;
-using View2 = FieldView<2>;
+template <typename F> using ViewS = FieldView<-1, F>;  // phi, chi
+template <typename F> using View0 = FieldView<0, F>;
+template <typename F> using View1 = FieldView<1, F>;
+template <typename F> using View2 = FieldView<2, F>;
 
 // Compiler-only memory fence between the phases of the particle update: the
 // LDS field values of one phase are read again in the next rather than held
@@ -239,9 +241,9 @@ using View2 = FieldView<2>;
 #define GEV_LDS_PHASE() asm volatile("" ::: "memory")
 
 // update_q, gevolution.hpp:570-678 (GRADIENT_ORDER 1); returns v^2/a^2
-template <bool POW2>
-__device__ __forceinline__ double update_q(double q[3], const double rd[3], ViewS phi,
-                                           ViewS chi, View0 B0, View1 B1, View2 B2,
+template <bool POW2, typename F>
+__device__ __forceinline__ double update_q(double q[3], const double rd[3], ViewS<F> phi,
+                                           ViewS<F> chi, View0<F> B0, View1<F> B1, View2<F> B2,
                                            int nfield, const double params[2], double dtau,
                                            double dx) {
     double gradphi[3], pgradB[3];
@@ -324,9 +326,9 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], View
 }
 
 // update_q_Newton, gevolution.hpp:709-776
-template <bool POW2>
-__device__ __forceinline__ double update_q_newton(double q[3], const double rd[3], ViewS psi,
-                                                  ViewS chi, int nfield, const double params[2],
+template <bool POW2, typename F>
+__device__ __forceinline__ double update_q_newton(double q[3], const double rd[3], ViewS<F> psi,
+                                                  ViewS<F> chi, int nfield, const double params[2],
                                                   double dtau, double dx) {
     double g[3];
     g[0] = (1. - rd[1]) * (1. - rd[2]) * (psi(1, 0, 0) - psi(0, 0, 0));
@@ -365,9 +367,10 @@ __device__ __forceinline__ double update_q_newton(double q[3], const double rd[3
 }
 
 // update_pos, gevolution.hpp:810-871
+template <typename F>
 __device__ __forceinline__ void update_pos(double pos[3], const double q[3], const double rd[3],
-                                           ViewS phi, ViewS chi, View0 B0,
-                                           View1 B1, View2 B2, int nfield,
+                                           ViewS<F> phi, ViewS<F> chi, View0<F> B0,
+                                           View1<F> B1, View2<F> B2, int nfield,
                                            const double params[2], double dtau) {
     double v[3];
     double v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
@@ -450,16 +453,16 @@ struct StepArgs {
 #define GEV_KICK_OCC
 #endif
 
-template <bool KICK, int QM, bool DRIFT, int PM, bool POW2>
+template <bool KICK, int QM, bool DRIFT, int PM, bool POW2, typename F>
 __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
     double *__restrict__ pos, double *__restrict__ q, int64_t cap, int N, int ntx, int nty, int ntz,
-    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const double *__restrict__ phi,
-    const double *__restrict__ chi, const double *__restrict__ Bi, StepArgs sa,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const F *__restrict__ phi,
+    const F *__restrict__ chi, const F *__restrict__ Bi, StepArgs sa,
     unsigned long long *__restrict__ vmax, uint32_t *__restrict__ count, uint32_t *__restrict__ cell,
     double *__restrict__ pos_out, double *__restrict__ q_out,
     const int64_t *__restrict__ id_in, int64_t *__restrict__ id_out) {
-    __shared__ double s_phi[Region<-1>::SIZE], s_chi[Region<-1>::SIZE];
-    __shared__ double s_b0[Region<0>::SIZE], s_b1[Region<1>::SIZE], s_b2[Region<2>::SIZE];
+    __shared__ F s_phi[Region<-1>::SIZE], s_chi[Region<-1>::SIZE];
+    __shared__ F s_b0[Region<0>::SIZE], s_b1[Region<1>::SIZE], s_b2[Region<2>::SIZE];
     __shared__ unsigned long long sh[TPB / 64];
 
     const int64_t nblk = (int64_t)ntx * nty * ntz;
@@ -527,7 +530,7 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
     };
     constexpr int PS = (Region<-1>::SIZE + TPB - 1) / TPB, P0 = (Region<0>::SIZE + TPB - 1) / TPB;
     constexpr int P1 = (Region<1>::SIZE + TPB - 1) / TPB, P2 = (Region<2>::SIZE + TPB - 1) / TPB;
-    double vp[PS], vc[PS], v0[P0], v1[P1], v2[P2];
+    F vp[PS], vc[PS], v0[P0], v1[P1], v2[P2];
 #pragma unroll
     for (int k = 0; k < PS; k++) {
         const int i = threadIdx.x + k * TPB;
@@ -600,10 +603,10 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
             ci[i] = (int)fl;
             rd[i] = t - fl;
         }
-        const ViewS fphi = view<-1>(s_phi, ci, x0, y0, z0), fchi = view<-1>(s_chi, ci, x0, y0, z0);
-        const View0 fb0 = view<0>(s_b0, ci, x0, y0, z0);
-        const View1 fb1 = view<1>(s_b1, ci, x0, y0, z0);
-        const View2 fb2 = view<2>(s_b2, ci, x0, y0, z0);
+        const ViewS<F> fphi = view<-1>(s_phi, ci, x0, y0, z0), fchi = view<-1>(s_chi, ci, x0, y0, z0);
+        const View0<F> fb0 = view<0>(s_b0, ci, x0, y0, z0);
+        const View1<F> fb1 = view<1>(s_b1, ci, x0, y0, z0);
+        const View2<F> fb2 = view<2>(s_b2, ci, x0, y0, z0);
         if (KICK) {
             double r;
             if (QM == 0)
@@ -673,21 +676,28 @@ __global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
     }
 }
 
-template <bool KICK, int QM, bool DRIFT, int PM>
-void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
+template <bool KICK, int QM, bool DRIFT, int PM, typename F>
+void launch_update_f(gev_ctx *ctx, const StepArgs &sa) {
     const int N = ctx->N;
     const int ntx = (N + KTX - 1) / KTX, nty = (N + KTY - 1) / KTY, ntz = (N + KTZ - 1) / KTZ;
     const int64_t nblk = (int64_t)ntx * nty * ntz;
+    const F *phi = gev::fld<F>(ctx->phi), *chi = gev::fld<F>(ctx->chi), *Bi = gev::fld<F>(ctx->Bi);
     if ((N & (N - 1)) == 0)
-        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, true>), dim3((unsigned)nblk), dim3(TPB), 0,
+        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, true, F>), dim3((unsigned)nblk), dim3(TPB), 0,
                            ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell,
-                           ctx->pos2, ctx->q2, ctx->id, ctx->id2);
+                           phi, chi, Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->pos2, ctx->q2, ctx->id,
+                           ctx->id2);
     else
-        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, false>), dim3((unsigned)nblk), dim3(TPB), 0,
+        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, false, F>), dim3((unsigned)nblk), dim3(TPB), 0,
                            ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           ctx->phi, ctx->chi, ctx->Bi, sa, ctx->red_u, ctx->count, ctx->cell,
-                           ctx->pos2, ctx->q2, ctx->id, ctx->id2);
+                           phi, chi, Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->pos2, ctx->q2, ctx->id,
+                           ctx->id2);
+}
+
+template <bool KICK, int QM, bool DRIFT, int PM>
+void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
+    if (ctx->fp32) launch_update_f<KICK, QM, DRIFT, PM, float>(ctx, sa);
+    else launch_update_f<KICK, QM, DRIFT, PM, double>(ctx, sa);
 }
 
 }  // namespace

@@ -33,7 +33,9 @@ __device__ __forceinline__ int64_t L(int N, int x, int y, int z) {
 }
 
 // prepareFTsource (2), gevolution.hpp:169-192: the new source value at site i
-__device__ __forceinline__ double prep_scalar(const double *__restrict__ phi, double chi_i, double src_i,
+// (phi stored as F: double, or float in the FP32 form; arithmetic in double)
+template <typename F>
+__device__ __forceinline__ double prep_scalar(const F *__restrict__ phi, double chi_i, double src_i,
                                               const Nb &c, int64_t i, int N, double bg, double coeff,
                                               double coeff2, double coeff3) {
     const double p = phi[i];
@@ -51,7 +53,8 @@ __device__ __forceinline__ double prep_scalar(const double *__restrict__ phi, do
 
 // prepareFTsource (1), gevolution.hpp:56-147: S[6] (order 00,01,02,11,12,22)
 // from the deposited T[6] at site i
-__device__ __forceinline__ void prep_tensor(const double *__restrict__ phi, const double T[6], const Nb &c,
+template <typename F>
+__device__ __forceinline__ void prep_tensor(const F *__restrict__ phi, const double T[6], const Nb &c,
                                             int64_t i, int N, double coeff, double S[6]) {
     const double p = phi[i];
     const double px = phi[L(N, c.xp, c.y, c.z)], mx = phi[L(N, c.xm, c.y, c.z)];

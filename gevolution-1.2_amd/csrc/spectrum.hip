@@ -20,8 +20,10 @@ namespace {
 constexpr int TPB = 256;
 constexpr int NQ = 5;  // kbin, kscatter, power, pscatter, occupation
 
-struct PkArgs {
-    const double2 *f1, *f2;
+// C: the k-field element (double2, or float2 in the FP32 form); the products
+// and sums are FP64 either way
+template <typename C> struct PkArgsT {
+    const C *f1, *f2;
     int64_t nkv;
     int ncomp, mode;  // mode 0: components (comp1, comp2); 1: symmetric tensor; 2: sum over components
     int comp1, comp2;
@@ -31,11 +33,15 @@ struct PkArgs {
     int64_t rows_per_block;
     double *partial;  // gridDim.x * numbins * NQ
 };
+using PkArgs = PkArgsT<double2>;
 
 // real part of a * conj(b), as the reference's complex product evaluates it
 __device__ __forceinline__ double pre(double2 a, double2 b) { return a.x * b.x - a.y * -b.y; }
+__device__ __forceinline__ double pre(float2 a, float2 b) {
+    return pre(make_double2(a.x, a.y), make_double2(b.x, b.y));
+}
 
-__device__ __forceinline__ double cross(const PkArgs &A, int64_t i) {
+template <typename C> __device__ __forceinline__ double cross(const PkArgsT<C> &A, int64_t i) {
     if (A.mode == 0) return pre(A.f1[A.comp1 * A.nkv + i], A.f2[A.comp2 * A.nkv + i]);
     if (A.mode == 1) {
         double p = pre(A.f1[A.nkv + i], A.f2[A.nkv + i]);
@@ -52,8 +58,8 @@ __device__ __forceinline__ double cross(const PkArgs &A, int64_t i) {
     return p;
 }
 
-template <int MAXH>
-__global__ __launch_bounds__(TPB) void k_spectrum_rows(PkArgs A) {
+template <int MAXH, typename C>
+__global__ __launch_bounds__(TPB) void k_spectrum_rows(PkArgsT<C> A) {
     extern __shared__ double hist[];  // numbins * NQ
     __shared__ int s_bin[MAXH];
     __shared__ double s_q[NQ - 1][MAXH];
@@ -194,10 +200,33 @@ int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mo
     {
         ScopedTimer t(c, "power_spectrum");
         const size_t shm = (size_t)numbins * NQ * 8;
-        if (N / 2 + 1 <= 257)
-            hipLaunchKernelGGL(k_spectrum_rows<257>, dim3(nblk), dim3(TPB), shm, c->stream, A);
-        else
-            hipLaunchKernelGGL(k_spectrum_rows<513>, dim3(nblk), dim3(TPB), shm, c->stream, A);
+        if (c->fp32) {
+            // the same arguments over float2 fields
+            PkArgsT<float2> B;
+            B.f1 = cfld<float2>(A.f1);
+            B.f2 = cfld<float2>(A.f2);
+            B.nkv = A.nkv;
+            B.ncomp = A.ncomp;
+            B.mode = A.mode;
+            B.comp1 = A.comp1;
+            B.comp2 = A.comp2;
+            B.typek2 = A.typek2;
+            B.sinc = A.sinc;
+            B.k2max = A.k2max;
+            B.N = A.N;
+            B.P = A.P;
+            B.numbins = A.numbins;
+            B.rows_per_block = A.rows_per_block;
+            B.partial = A.partial;
+            if (N / 2 + 1 <= 257)
+                hipLaunchKernelGGL((k_spectrum_rows<257, float2>), dim3(nblk), dim3(TPB), shm, c->stream, B);
+            else
+                hipLaunchKernelGGL((k_spectrum_rows<513, float2>), dim3(nblk), dim3(TPB), shm, c->stream, B);
+        } else if (N / 2 + 1 <= 257) {
+            hipLaunchKernelGGL((k_spectrum_rows<257, double2>), dim3(nblk), dim3(TPB), shm, c->stream, A);
+        } else {
+            hipLaunchKernelGGL((k_spectrum_rows<513, double2>), dim3(nblk), dim3(TPB), shm, c->stream, A);
+        }
         hipLaunchKernelGGL(k_spectrum_reduce, dim3((numbins * NQ + TPB - 1) / TPB), dim3(TPB), 0, c->stream,
                            (const double *)dpart, nblk, numbins, (double *)dtot);
     }

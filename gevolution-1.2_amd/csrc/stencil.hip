@@ -12,22 +12,24 @@ namespace {
 using namespace gev_prep;
 constexpr int TPB = 256;
 
+// F: the field element (double, or float in the FP32 form); the stencils
+// evaluate in double either way.
+
 // prepareFTsource (2), gevolution.hpp:169-192
-__global__ __launch_bounds__(TPB) void k_prepare_scalar(const double *__restrict__ phi,
-                                                        const double *__restrict__ chi,
-                                                        double *source, int N, int64_t n3v,
-                                                        double bg, double coeff, double coeff2,
-                                                        double coeff3) {
+template <typename F>
+__global__ __launch_bounds__(TPB) void k_prepare_scalar(const F *__restrict__ phi, const F *__restrict__ chi,
+                                                        F *source, int N, int64_t n3v, double bg, double coeff,
+                                                        double coeff2, double coeff3) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= n3v) return;
     const int64_t r = i / N;
     const Nb c = nb_of((int)(i % N), (int)(r % N), (int)(r / N), N);
-    source[i] = prep_scalar(phi, chi[i], source[i], c, i, N, bg, coeff, coeff2, coeff3);
+    source[i] = (F)prep_scalar(phi, (double)chi[i], (double)source[i], c, i, N, bg, coeff, coeff2, coeff3);
 }
 
 // prepareFTsource (1), gevolution.hpp:56-147
-__global__ __launch_bounds__(TPB) void k_prepare_tensor(const double *__restrict__ phi,
-                                                        double *Sij, int N, int64_t n3v,
+template <typename F>
+__global__ __launch_bounds__(TPB) void k_prepare_tensor(const F *__restrict__ phi, F *Sij, int N, int64_t n3v,
                                                         double coeff) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= n3v) return;
@@ -38,7 +40,7 @@ __global__ __launch_bounds__(TPB) void k_prepare_tensor(const double *__restrict
     for (int k = 0; k < 6; k++) T[k] = Sij[k * n3v + i];
     prep_tensor(phi, T, c, i, N, coeff, S);
 #pragma unroll
-    for (int k = 0; k < 6; k++) Sij[k * n3v + i] = S[k];
+    for (int k = 0; k < 6; k++) Sij[k * n3v + i] = (F)S[k];
 }
 
 // deterministic two-level sum: fixed per-block tree, then one block over
@@ -58,13 +60,14 @@ __device__ __forceinline__ double block_sum(double v) {
     return r;
 }
 
-__global__ __launch_bounds__(TPB) void k_sum_partial(const double *__restrict__ f, int64_t n,
+template <typename F>
+__global__ __launch_bounds__(TPB) void k_sum_partial(const F *__restrict__ f, int64_t n,
                                                      double *__restrict__ part) {
     const int64_t base = (int64_t)blockIdx.x * TPB * SUM_ITEMS;
     double v = 0.;
     for (int k = 0; k < SUM_ITEMS; k++) {
         int64_t i = base + (int64_t)k * TPB + threadIdx.x;
-        if (i < n) v += f[i];
+        if (i < n) v += (double)f[i];
     }
     double r = block_sum(v);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -83,33 +86,54 @@ __global__ __launch_bounds__(TPB) void k_fill_zero(double2 *__restrict__ p, int6
     if (i < n2) p[i] = make_double2(0., 0.);
 }
 
+// element-wise copy between the two forms' field elements
+template <typename S, typename D>
+__global__ __launch_bounds__(TPB) void k_convert(const S *__restrict__ src, D *__restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) dst[i] = (D)src[i];
+}
+
 }  // namespace
 
 namespace gev {
 
 int launch_prepare_scalar(gev_ctx *ctx, double bg, double c1, double c2, double c3) {
     ScopedTimer t(ctx, "prepare_ft_source_scalar");
-    hipLaunchKernelGGL(k_prepare_scalar, dim3(grid_for(n3(ctx), TPB)), dim3(TPB), 0, ctx->stream,
-                       ctx->phi, ctx->chi, ctx->source, ctx->N, n3(ctx), bg, c1, c2, c3);
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_prepare_scalar<float>, dim3(grid_for(n3(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           fld<float>(ctx->phi), fld<float>(ctx->chi), fld<float>(ctx->source), ctx->N, n3(ctx), bg,
+                           c1, c2, c3);
+    else
+        hipLaunchKernelGGL(k_prepare_scalar<double>, dim3(grid_for(n3(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           ctx->phi, ctx->chi, ctx->source, ctx->N, n3(ctx), bg, c1, c2, c3);
     return check_hip(ctx, hipGetLastError(), "prepare scalar launch");
 }
 
 int launch_prepare_tensor(gev_ctx *ctx, double coeff) {
     ScopedTimer t(ctx, "prepare_ft_source_tensor");
-    hipLaunchKernelGGL(k_prepare_tensor, dim3(grid_for(n3(ctx), TPB)), dim3(TPB), 0, ctx->stream,
-                       ctx->phi, ctx->Sij, ctx->N, n3(ctx), coeff);
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_prepare_tensor<float>, dim3(grid_for(n3(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           fld<float>(ctx->phi), fld<float>(ctx->Sij), ctx->N, n3(ctx), coeff);
+    else
+        hipLaunchKernelGGL(k_prepare_tensor<double>, dim3(grid_for(n3(ctx), TPB)), dim3(TPB), 0, ctx->stream,
+                           ctx->phi, ctx->Sij, ctx->N, n3(ctx), coeff);
     return check_hip(ctx, hipGetLastError(), "prepare tensor launch");
 }
 
-// sums count doubles into ctx->red_d[0]; copies to *out_host if non-null (sync)
+// sums count field elements (double, or float in the FP32 form) into
+// ctx->red_d[0] in FP64; copies to *out_host if non-null (sync)
 int launch_field_sum(gev_ctx *ctx, const double *f, int64_t count, double *out_host) {
     const int64_t per = (int64_t)TPB * SUM_ITEMS;
     const int64_t nblk = (count + per - 1) / per;
     // partials live in the reduction scratch after slot 0
     {
         ScopedTimer t(ctx, "field_sum");
-        hipLaunchKernelGGL(k_sum_partial, dim3((unsigned)nblk), dim3(TPB), 0, ctx->stream, f, count,
-                           ctx->red_d + 8);
+        if (ctx->fp32)
+            hipLaunchKernelGGL(k_sum_partial<float>, dim3((unsigned)nblk), dim3(TPB), 0, ctx->stream,
+                               fld<float>(f), count, ctx->red_d + 8);
+        else
+            hipLaunchKernelGGL(k_sum_partial<double>, dim3((unsigned)nblk), dim3(TPB), 0, ctx->stream, f, count,
+                               ctx->red_d + 8);
         hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(TPB), 0, ctx->stream, ctx->red_d + 8, nblk,
                            ctx->red_d);
     }
@@ -135,6 +159,23 @@ int launch_fill_zero(gev_ctx *ctx, void *ptr, size_t bytes) {
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n2, TPB)), dim3(TPB), 0, ctx->stream,
                        (double2 *)ptr, n2);
     return check_hip(ctx, hipGetLastError(), "fill launch");
+}
+
+// n reals from src (float if src32, else double) to dst (float if dst32),
+// rounded to nearest when narrowing; on ctx's stream
+int launch_convert(gev_ctx *ctx, const void *src, bool src32, void *dst, bool dst32, int64_t n) {
+    if (n <= 0) return 0;
+    const dim3 g(grid_for(n, TPB)), b(TPB);
+    if (src32 && dst32)
+        hipLaunchKernelGGL((k_convert<float, float>), g, b, 0, ctx->stream, (const float *)src, (float *)dst, n);
+    else if (src32)
+        hipLaunchKernelGGL((k_convert<float, double>), g, b, 0, ctx->stream, (const float *)src, (double *)dst, n);
+    else if (dst32)
+        hipLaunchKernelGGL((k_convert<double, float>), g, b, 0, ctx->stream, (const double *)src, (float *)dst, n);
+    else
+        hipLaunchKernelGGL((k_convert<double, double>), g, b, 0, ctx->stream, (const double *)src, (double *)dst,
+                           n);
+    return check_hip(ctx, hipGetLastError(), "convert launch");
 }
 
 }  // namespace gev

@@ -47,12 +47,16 @@ EXPORTED = [
     "gev_initialize_particle_positions", "gev_displace_particles_ic", "gev_initialize_q_ic",
     "gev_generate_ic_basic", "gev_extract_cross_spectrum", "gev_extract_power_spectrum",
     "gev_write_power_spectrum", "gev_hibernate", "gev_restart", "gev_run_cycles_recorded",
-    "gev_write_background",
+    "gev_write_background", "gev_copy_state",
 ]
 
 
+# forms of a box (include/gev_pm.h)
+FORM_FP64, FORM_FP32 = 0, 1
+
+
 class BoxDesc(ctypes.Structure):
-    _fields_ = [("ngrid", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+    _fields_ = [("ngrid", ctypes.c_int32), ("form", ctypes.c_int32),
                 ("npart", ctypes.c_int64), ("stream", _p)]
 
 
@@ -107,6 +111,7 @@ def lib():
         L.gev_download_filing.argtypes = [_p, _p, _p]
         L.gev_upload_field.argtypes = [_p, _i, _p]
         L.gev_download_field.argtypes = [_p, _i, _p]
+        L.gev_copy_state.argtypes = [_p, _p]
         L.gev_projection_init.argtypes = [_p, _i]
         L.gev_projection_T00_project.argtypes = [_p, _d, _i, _d]
         L.gev_projection_Tij_project.argtypes = [_p, _d, _i, _d]
@@ -189,22 +194,24 @@ def rungekutta4bg(a, fourpiG, cosmo, dtau):
     return x.value
 
 
-def footprint_bytes(ngrid, npart):
-    d = BoxDesc(ngrid, 0, npart, None)
+def footprint_bytes(ngrid, npart, form=FORM_FP64):
+    d = BoxDesc(ngrid, form, npart, None)
     return lib().gev_ctx_footprint_bytes(ctypes.byref(d))
 
 
 class Box:
     """One simulation box resident on the current GPU (a gev_ctx)."""
 
-    def __init__(self, ngrid, npart, stream=None):
+    def __init__(self, ngrid, npart, stream=None, form=FORM_FP64):
         self.N = int(ngrid)
         self.cap = int(npart)
+        self.form = int(form)
         self._c = _p()
-        d = BoxDesc(self.N, 0, self.cap, stream)
+        d = BoxDesc(self.N, self.form, self.cap, stream)
         rc = lib().gev_ctx_create(ctypes.byref(d), ctypes.byref(self._c))
         if rc != 0:
-            raise GevError("gev_ctx_create(N=%d, npart=%d) failed with %d" % (self.N, self.cap, rc))
+            raise GevError("gev_ctx_create(N=%d, npart=%d, form=%d) failed with %d"
+                           % (self.N, self.cap, self.form, rc))
 
     # -- plumbing
     def _chk(self, rc, what):
@@ -282,6 +289,11 @@ class Box:
         a = np.zeros(shp, dtype=dt)
         self._chk(lib().gev_download_field(self._c, f, _ptr(a)), "download_field")
         return a
+
+    def copy_state_from(self, other):
+        """This box takes other's particles, fields and carried state,
+        converted to this box's form (gev_copy_state)."""
+        self._chk(lib().gev_copy_state(other._c, self._c), "copy_state")
 
     # -- the reference call surface
     def projection_init(self, f):

@@ -73,9 +73,16 @@ enum { GEV_FFT_FORWARD = 0, GEV_FFT_BACKWARD = 1 };
 enum { GEV_Q_GR = 0, GEV_Q_NEWTON = 1 };     /* update_q, update_q_Newton     */
 enum { GEV_POS_GR = 0, GEV_POS_NEWTON = 1 }; /* update_pos, update_pos_Newton */
 
+/* forms of a box (gev_box_desc.form) */
+enum {
+    GEV_FORM_FP64 = 0, /* plain FP64: the drop-in                                  */
+    GEV_FORM_FP32 = 1  /* precision form: FP32 field storage and FFT passes; FP64
+                          particles, background, deposit/stencil/solve arithmetic */
+};
+
 typedef struct {
     int32_t ngrid;      /* N: lattice points per dimension (2^a 3^b, 8..1024)  */
-    int32_t reserved0;
+    int32_t form;       /* GEV_FORM_FP64 (0) or GEV_FORM_FP32                  */
     int64_t npart;      /* particle capacity                                   */
     void *stream;       /* hipStream_t to run on; NULL = a new stream          */
 } gev_box_desc;
@@ -108,8 +115,18 @@ int64_t gev_particle_count(const gev_ctx *ctx);
    in storage order (n uint32) as filed by the last moveParticles/upload. */
 int gev_download_filing(gev_ctx *ctx, uint32_t *cell_count, uint32_t *particle_cell);
 
+/* Host arrays are FP64 in both forms; an FP32 context rounds uploads to float
+   and widens downloads. */
 int gev_upload_field(gev_ctx *ctx, gev_field f, const double *host);
 int gev_download_field(gev_ctx *ctx, gev_field f, double *host);
+
+/* Carries the whole state of src into dst (same ngrid, same device, capacity
+   for src's particles), converting between the forms: particles, every real
+   and k-space field (BiFT, the carried vector state, included) and the
+   species mass; dst refiles its particles.  The initial-condition generator
+   runs on an FP64 context; this brings its result into an FP32 one.  No
+   counterpart in the reference, which has one precision per build. */
+int gev_copy_state(gev_ctx *src, gev_ctx *dst);
 
 /* ---- hot path: one entry per reference call ----------------------------- */
 

@@ -44,3 +44,18 @@ def test_host_background_matches_oracle(orc):
         for fpg in [0.0171, 0.7]:
             assert gevpm.hconf(a, fpg, cg) == orc.hconf(a, fpg, co)
             assert gevpm.rungekutta4bg(a, fpg, cg, 0.01) == orc.rk4bg(a, fpg, co, 0.01)
+
+
+def test_footprint_of_both_forms():
+    """gev_ctx_footprint_bytes is host arithmetic (no GPU): the FP32 form
+    halves the field and k-space storage and keeps the FP64 particles."""
+    import gevpm
+    N, cap = 512, 512 ** 3
+    f64 = gevpm.footprint_bytes(N, cap, gevpm.FORM_FP64)
+    f32 = gevpm.footprint_bytes(N, cap, gevpm.FORM_FP32)
+    fields64 = 12 * N ** 3 * 8 + 11 * (N // 2 + 1) * N * N * 16
+    particles = cap * (2 * (6 * 8 + 8) + 2 * 4)
+    assert f64 >= fields64 + particles
+    assert f64 - f32 >= fields64 // 2 - N * 8  # every field element halves
+    assert f32 >= fields64 // 2 + particles
+    assert gevpm.footprint_bytes(N, cap, 5) < 0  # unknown form
