@@ -39,36 +39,38 @@ HBM_PEAK_GBS = 8000.0
 C_LIGHT = 2997.92458
 
 
-def algorithmic_bytes(name, N, Np):
-    """Bytes each kernel must move at least once per launch (FP64 form):
-    every input read once, every output written once."""
+def algorithmic_bytes(name, N, Np, f=8):
+    """Bytes each kernel must move at least once per launch: every input read
+    once, every output written once.  f: bytes of a real field element (8 in
+    the FP64 form, 4 in the FP32 form; particles are FP64 in both)."""
     n3 = N ** 3
     nk = N * N * (N // 2 + 1)
+    c = 2 * f  # a complex k-field element
     table = {
-        "kick_drift": 96 * Np + 5 * 8 * n3,                 # pos,q r+w; phi, chi, B(3) read
-        "update_vel": 72 * Np + 5 * 8 * n3,
-        "move_particles": 72 * Np + 5 * 8 * n3,
-        "projection_T00_Tij_project": 48 * Np + 8 * 8 * n3,  # pos,q read; phi read; T00 + Tij(6) written
-        "projection_T00_project": 48 * Np + 2 * 8 * n3,
-        "projection_Tij_project": 48 * Np + 7 * 8 * n3,
-        "prepare_ft_source_tensor": 13 * 8 * n3,            # phi + Tij(6) read, Sij(6) written
-        "prepare_ft_source_scalar": 4 * 8 * n3,             # phi, chi, source read, source written
-        "fft_r2c_x": 8 * n3 + 16 * nk,
-        "fft_c2r_x": 16 * nk + 8 * n3,
-        "fft_c2c_y": 32 * nk,
-        "fft_c2c_z": 32 * nk,
-        "fft_c2c_y_inv": 32 * nk,
-        "fft_c2c_z_inv": 32 * nk,
-        "solve_modified_poisson_ft": 32 * nk,
-        "project_scalar_evolve_vector": 13 * 16 * nk,       # SijFT(6)+BiFT(3) read, chiFT+BiFT(3) written
+        "kick_drift": 96 * Np + 5 * f * n3,                 # pos,q r+w; phi, chi, B(3) read
+        "update_vel": 72 * Np + 5 * f * n3,
+        "move_particles": 72 * Np + 5 * f * n3,
+        "projection_T00_Tij_project": 48 * Np + 8 * f * n3,  # pos,q read; phi read; T00 + Tij(6) written
+        "projection_T00_project": 48 * Np + 2 * f * n3,
+        "projection_Tij_project": 48 * Np + 7 * f * n3,
+        "prepare_ft_source_tensor": 13 * f * n3,            # phi + Tij(6) read, Sij(6) written
+        "prepare_ft_source_scalar": 4 * f * n3,             # phi, chi, source read, source written
+        "fft_r2c_x": f * n3 + c * nk,
+        "fft_c2r_x": c * nk + f * n3,
+        "fft_c2c_y": 2 * c * nk,
+        "fft_c2c_z": 2 * c * nk,
+        "fft_c2c_y_inv": 2 * c * nk,
+        "fft_c2c_z_inv": 2 * c * nk,
+        "solve_modified_poisson_ft": 2 * c * nk,
+        "project_scalar_evolve_vector": 13 * c * nk,        # SijFT(6)+BiFT(3) read, chiFT+BiFT(3) written
         # fused passes (fftsolve.hip)
-        "prepare_tensor_fft_r2c_x": 7 * 8 * n3 + 6 * 16 * nk,   # phi + Tij(6) read, 6 k rows written
-        "prepare_scalar_fft_r2c_x": 3 * 8 * n3 + 16 * nk,       # phi, chi, source read, 1 k row written
-        "fft_z_poisson_z": 2 * 16 * nk,                         # y-passed source in, z-inverted phi out
-        "fft_z_tensor_z": 16 * 16 * nk,                         # Sij(6)+BiFT(3) in, BiFT(3)+chi+B(3) out
-        "fft_z_tensor_scalar_z": 7 * 16 * nk,
-        "fft_z_vector_z": 9 * 16 * nk,
-        "field_sum": 8 * n3,
+        "prepare_tensor_fft_r2c_x": 7 * f * n3 + 6 * c * nk,   # phi + Tij(6) read, 6 k rows written
+        "prepare_scalar_fft_r2c_x": 3 * f * n3 + c * nk,       # phi, chi, source read, 1 k row written
+        "fft_z_poisson_z": 2 * c * nk,                         # y-passed source in, z-inverted phi out
+        "fft_z_tensor_z": 16 * c * nk,                         # Sij(6)+BiFT(3) in, BiFT(3)+chi+B(3) out
+        "fft_z_tensor_scalar_z": 7 * c * nk,
+        "fft_z_vector_z": 9 * c * nk,
+        "field_sum": f * n3,
         "filing_scan": 8 * n3,
         "filing_scatter": 12 * Np,
         "filing_sort": 4 * n3,
@@ -94,9 +96,10 @@ def algorithmic_bytes(name, N, Np):
 #   FFT -> chi 2c N^3,  FFT -> B x3  6c N^3
 #   kick + drift             96 Np + 40 N^3
 #   cell key + histogram     8 N^3
-def stage_bytes(N, Np):
-    c = 8.0 * (1.0 + 2.0 / N)
-    return int(144 * Np + 248 * N ** 3 + 37 * c * N ** 3)
+# (f = 8; in the FP32 form f = 4 for every field term, the cell key stays 8 N^3)
+def stage_bytes(N, Np, f=8):
+    c = f * (1.0 + 2.0 / N)
+    return int(144 * Np + (30 * f + 8) * N ** 3 + 37 * c * N ** 3)
 
 
 # Minimum bytes of one GR-parabolic cycle (FP64 fields and particles): every
@@ -200,6 +203,19 @@ def cpu_baseline(N_sample, cycles, threads):
     return N ** 3 * cycles / dt, dt
 
 
+def kernel_breakdown(kernels, steps, N, Np, f):
+    """Per-kernel calls per step, mean duration, share and achieved GB/s."""
+    out = {}
+    total_ms = sum(ms for _, ms in kernels.values())
+    for name, (calls, ms) in sorted(kernels.items(), key=lambda kv: -kv[1][1]):
+        avg = ms / calls
+        b = algorithmic_bytes(name, N, Np, f)
+        out[name] = {"calls_per_step": calls / steps, "ms_per_launch": round(avg, 4),
+                     "share": round(ms / total_ms, 4),
+                     "GBps": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+    return out
+
+
 def timed_cycles(box, sim, st, warmup, steps, timers):
     """warmup untimed cycles, then steps timed ones; (seconds, per-kernel table)."""
     box.run_cycles(sim, st, warmup)
@@ -242,7 +258,6 @@ def precision_form_report(gev, box, st, box32, st32, sim, args, ms64):
     p32 = delta_spectrum(gev, box32, st32.a, nb)
     low = (p64[4] > 0) & (p64[0] < 0.5 * np.pi * N)
     pk_err = float(np.max(np.abs(p32[1][low] - p64[1][low]) / np.abs(p64[1][low])))
-    top = sorted(k32.items(), key=lambda kv: -kv[1][1])[:8]
     print("FP32 form: %.3f ms/cycle (%.2fx), phi error %.2e, P(k) error %.2e"
           % (ms32, ms64 / ms32, phi_err, pk_err), file=sys.stderr, flush=True)
     return {"form": "fp32 fields and FFT passes; fp64 particles, background, deposit sums, stencil and "
@@ -252,7 +267,8 @@ def precision_form_report(gev, box, st, box32, st32, sim, args, ms64):
             "phi_max_rel_err": phi_err, "pk_max_rel_err_below_half_nyquist": pk_err,
             "pk_bins_compared": int(low.sum()), "cycles_from_same_state": args.warmup + args.steps,
             "z_at_comparison": round(1.0 / st.a - 1.0, 4),
-            "kernels_ms_per_launch": {n: round(ms / c, 4) for n, (c, ms) in top}}
+            "step_bytes": stage_bytes(N, N ** 3, 4),
+            "kernels": kernel_breakdown(k32, args.steps, N, N ** 3, 4) if k32 else {}}
 
 
 def main():
@@ -337,13 +353,7 @@ def main():
     roofline = None
     breakdown = {}
     if kernels:
-        total_ms = sum(ms for _, ms in kernels.values())
-        for name, (calls, ms) in sorted(kernels.items(), key=lambda kv: -kv[1][1]):
-            avg = ms / calls
-            b = algorithmic_bytes(name, N, Np)
-            breakdown[name] = {"calls_per_step": calls / args.steps, "ms_per_launch": round(avg, 4),
-                               "share": round(ms / total_ms, 4),
-                               "GBps": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+        breakdown = kernel_breakdown(kernels, args.steps, N, Np, 8)
         dom, (calls, ms) = max(kernels.items(), key=lambda kv: kv[1][1])
         avg_s = ms / calls * 1e-3
         b = algorithmic_bytes(dom, N, Np)

@@ -73,13 +73,17 @@ template <int KIND> struct Kind {
     static constexpr bool QE = (T00 && KIND != 5) || TIJ;  // needs e = sqrt(q^2 + a^2)
 };
 
-// per-particle record: everything a target needs from one particle
-struct Rec {
-    double up0, up1, up2;      // (pos - source corner) / dx
-    double e, f00;             // T00: e and f (gevolution.hpp:989-991)
-    double fij;                // Tij: f (gevolution.hpp:1238)
-    double w0, w1, w2;         // Tij: m q_i q_i / e   |  T0i: m q_i
-    double w01, w02, w12;      // Tij: m q_i q_j / e
+// per-particle record: everything a target needs from one particle.  R is
+// the arithmetic type of the records, the partial sums and the accumulators:
+// double in the FP64 form, float in the FP32 form (whose fields are float).
+// A record is always evaluated in double from the FP64 particle and then
+// rounded to R.
+template <typename R> struct Rec {
+    R up0, up1, up2;      // (pos - source corner) / dx
+    R e, f00;             // T00: e and f (gevolution.hpp:989-991)
+    R fij;                // Tij: f (gevolution.hpp:1238)
+    R w0, w1, w2;         // Tij: m q_i q_i / e   |  T0i: m q_i
+    R w01, w02, w12;      // Tij: m q_i q_j / e
 };
 
 // position and momentum of one particle
@@ -105,10 +109,10 @@ __device__ __forceinline__ PQ load_pq(const double *__restrict__ pos, const doub
 }
 
 template <int KIND, bool USE_PHI, bool POW2>
-__device__ __forceinline__ Rec make_rec(const PQ &v, double rx, double ry, double rz, double dx, double Nd,
-                                        double a, double mass_t) {
+__device__ __forceinline__ Rec<double> make_rec(const PQ &v, double rx, double ry, double rz, double dx, double Nd,
+                                                double a, double mass_t) {
     using K = Kind<KIND>;
-    Rec r;
+    Rec<double> r;
     r.up0 = over_dx<POW2>(v.x - rx, dx, Nd);
     r.up1 = over_dx<POW2>(v.y - ry, dx, Nd);
     r.up2 = over_dx<POW2>(v.z - rz, dx, Nd);
@@ -143,82 +147,86 @@ __device__ __forceinline__ Rec make_rec(const PQ &v, double rx, double ry, doubl
 }
 
 // LDS image of the records (structure of arrays)
-template <int KIND> struct Stage {
-    double up0[CAP], up1[CAP], up2[CAP];
-    double e[Kind<KIND>::T00 ? CAP : 1], f00[Kind<KIND>::T00 ? CAP : 1];
-    double fij[Kind<KIND>::TIJ ? CAP : 1];
-    double w0[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
-    double w1[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
-    double w2[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
-    double w01[Kind<KIND>::TIJ ? CAP : 1], w02[Kind<KIND>::TIJ ? CAP : 1], w12[Kind<KIND>::TIJ ? CAP : 1];
+template <int KIND, typename R> struct Stage {
+    R up0[CAP], up1[CAP], up2[CAP];
+    R e[Kind<KIND>::T00 ? CAP : 1], f00[Kind<KIND>::T00 ? CAP : 1];
+    R fij[Kind<KIND>::TIJ ? CAP : 1];
+    R w0[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
+    R w1[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
+    R w2[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
+    R w01[Kind<KIND>::TIJ ? CAP : 1], w02[Kind<KIND>::TIJ ? CAP : 1], w12[Kind<KIND>::TIJ ? CAP : 1];
 
-    __device__ __forceinline__ void put(int i, const Rec &r) {
+    // stores a record evaluated in double, rounded to R
+    __device__ __forceinline__ void put(int i, const Rec<double> &r) {
         using K = Kind<KIND>;
-        up0[i] = r.up0;
-        up1[i] = r.up1;
-        up2[i] = r.up2;
+        up0[i] = (R)r.up0;
+        up1[i] = (R)r.up1;
+        up2[i] = (R)r.up2;
         if (K::T00) {
-            e[i] = r.e;
-            f00[i] = r.f00;
+            e[i] = (R)r.e;
+            f00[i] = (R)r.f00;
         }
-        if (K::TIJ) fij[i] = r.fij;
+        if (K::TIJ) fij[i] = (R)r.fij;
         if (K::TIJ || K::T0I) {
-            w0[i] = r.w0;
-            w1[i] = r.w1;
-            w2[i] = r.w2;
+            w0[i] = (R)r.w0;
+            w1[i] = (R)r.w1;
+            w2[i] = (R)r.w2;
         }
         if (K::TIJ) {
-            w01[i] = r.w01;
-            w02[i] = r.w02;
-            w12[i] = r.w12;
+            w01[i] = (R)r.w01;
+            w02[i] = (R)r.w02;
+            w12[i] = (R)r.w12;
         }
     }
-    __device__ __forceinline__ Rec get(int i) const {
+    __device__ __forceinline__ Rec<R> get(int i) const {
         using K = Kind<KIND>;
-        Rec r;
+        Rec<R> r;
         r.up0 = up0[i];
         r.up1 = up1[i];
         r.up2 = up2[i];
-        r.e = K::T00 ? e[i] : 0.;
-        r.f00 = K::T00 ? f00[i] : 0.;
-        r.fij = K::TIJ ? fij[i] : 0.;
-        r.w0 = (K::TIJ || K::T0I) ? w0[i] : 0.;
-        r.w1 = (K::TIJ || K::T0I) ? w1[i] : 0.;
-        r.w2 = (K::TIJ || K::T0I) ? w2[i] : 0.;
-        r.w01 = K::TIJ ? w01[i] : 0.;
-        r.w02 = K::TIJ ? w02[i] : 0.;
-        r.w12 = K::TIJ ? w12[i] : 0.;
+        r.e = K::T00 ? e[i] : (R)0;
+        r.f00 = K::T00 ? f00[i] : (R)0;
+        r.fij = K::TIJ ? fij[i] : (R)0;
+        r.w0 = (K::TIJ || K::T0I) ? w0[i] : (R)0;
+        r.w1 = (K::TIJ || K::T0I) ? w1[i] : (R)0;
+        r.w2 = (K::TIJ || K::T0I) ? w2[i] : (R)0;
+        r.w01 = K::TIJ ? w01[i] : (R)0;
+        r.w02 = K::TIJ ? w02[i] : (R)0;
+        r.w12 = K::TIJ ? w12[i] : (R)0;
         return r;
     }
 };
 
 // target-side values and the running per-source partial sums
-struct Target {
-    double ph, plq01, plq02, plq12, lnk0, lnk1, lnk2;
+template <typename R> struct Target {
+    R ph, plq01, plq02, plq12, lnk0, lnk1, lnk2;
 };
 
-struct Partial {
-    double lc, ld0, ld1, ld2, lo01, lo02, lo12, lv0, lv1, lv2;
-    __device__ __forceinline__ void clear() { lc = ld0 = ld1 = ld2 = lo01 = lo02 = lo12 = lv0 = lv1 = lv2 = 0.; }
+template <typename R> struct Partial {
+    R lc, ld0, ld1, ld2, lo01, lo02, lo12, lv0, lv1, lv2;
+    __device__ __forceinline__ void clear() {
+        lc = ld0 = ld1 = ld2 = lo01 = lo02 = lo12 = lv0 = lv1 = lv2 = (R)0;
+    }
 };
 
 // one particle's contribution to one target (expression order of
 // gevolution.hpp:995-1009, 1241-1272, 1107-1126)
-template <int KIND>
-__device__ __forceinline__ void add_particle(Partial &P, const Rec &r, const Target &T, int dxx, int dy,
+template <int KIND, typename R>
+__device__ __forceinline__ void add_particle(Partial<R> &P, const Rec<R> &r, const Target<R> &T, int dxx, int dy,
                                              int dz) {
     using K = Kind<KIND>;
-    const double W0 = dxx ? r.up0 : 1.0 - r.up0;
-    const double W1 = dy ? r.up1 : 1.0 - r.up1;
-    const double W2 = dz ? r.up2 : 1.0 - r.up2;
+    const R one = (R)1, quarter = (R)0.25;
+    const R W0 = dxx ? r.up0 : one - r.up0;
+    const R W1 = dy ? r.up1 : one - r.up1;
+    const R W2 = dz ? r.up2 : one - r.up2;
     if (K::T00) P.lc += W0 * W1 * W2 * (r.e + r.f00 * T.ph);
     if (K::TIJ) {
-        P.ld0 += r.w0 * W0 * W1 * W2 * (1. + r.fij * T.ph);
-        P.ld1 += r.w1 * W0 * W1 * W2 * (1. + r.fij * T.ph);
-        P.ld2 += r.w2 * W0 * W1 * W2 * (1. + r.fij * T.ph);
-        if (dxx == 0 && dy == 0) P.lo01 += r.w01 * W2 * (1. + r.fij * 0.25 * T.plq01);
-        if (dxx == 0 && dz == 0) P.lo02 += r.w02 * W1 * (1. + r.fij * 0.25 * T.plq02);
-        if (dy == 0 && dz == 0) P.lo12 += r.w12 * W0 * (1. + r.fij * 0.25 * T.plq12);
+        P.ld0 += r.w0 * W0 * W1 * W2 * (one + r.fij * T.ph);
+        P.ld1 += r.w1 * W0 * W1 * W2 * (one + r.fij * T.ph);
+        P.ld2 += r.w2 * W0 * W1 * W2 * (one + r.fij * T.ph);
+        if (dxx == 0 && dy == 0) P.lo01 += r.w01 * W2 * (one + r.fij * quarter * T.plq01);
+        if (dxx == 0 && dz == 0) P.lo02 += r.w02 * W1 * (one + r.fij * quarter * T.plq02);
+        if (dy == 0 && dz == 0) P.lo12 += r.w12 * W0 * (one + r.fij * quarter * T.plq12);
     }
     if (K::T0I) {
         if (dxx == 0) P.lv0 += r.w0 * W1 * W2;
@@ -227,13 +235,13 @@ __device__ __forceinline__ void add_particle(Partial &P, const Rec &r, const Tar
     }
 }
 
-struct Acc {
-    double s00, d0, d1, d2, o01, o02, o12, v0, v1, v2;
+template <typename R> struct Acc {
+    R s00, d0, d1, d2, o01, o02, o12, v0, v1, v2;
 };
 
-template <int KIND>
-__device__ __forceinline__ void add_source(Acc &A, const Partial &P, const Target &T, int dxx, int dy,
-                                           int dz, double mass_s) {
+template <int KIND, typename R>
+__device__ __forceinline__ void add_source(Acc<R> &A, const Partial<R> &P, const Target<R> &T, int dxx, int dy,
+                                           int dz, R mass_s) {
     using K = Kind<KIND>;
     if (K::T00) A.s00 += P.lc * mass_s;
     if (K::TIJ) {
@@ -262,14 +270,14 @@ template <int KIND> __device__ __forceinline__ bool feeds(int dxx, int dy, int d
 // reference's local cube; the conditional terms of add_particle / add_source
 // fold away).  c: the source cell's index in the tile's source region.
 // RANGE = false: the caller knows c is in the chunk [cb, ce) (one chunk).
-template <int KIND, int DX, int DY, int DZ, bool RANGE = true, typename ST>
-__device__ __forceinline__ void add_source_cell(Acc &A, const ST &st, const Target &T, int c, int cb, int ce,
+template <int KIND, int DX, int DY, int DZ, bool RANGE = true, typename ST, typename R>
+__device__ __forceinline__ void add_source_cell(Acc<R> &A, const ST &st, const Target<R> &T, int c, int cb, int ce,
                                                 uint32_t base, const uint32_t *s_off, const uint32_t *s_cnt,
-                                                double mass_s) {
+                                                R mass_s) {
     if (!feeds<KIND>(DX, DY, DZ) || (RANGE && (c < cb || c >= ce))) return;
     const uint32_t n = s_cnt[c];
     if (n == 0) return;
-    Partial P;
+    Partial<R> P;
     P.clear();
     const uint32_t r0 = s_off[c] - base;
 #pragma unroll 1
@@ -277,11 +285,10 @@ __device__ __forceinline__ void add_source_cell(Acc &A, const ST &st, const Targ
     add_source<KIND>(A, P, T, DX, DY, DZ, mass_s);
 }
 
-// the target's sums, written once (rounded to F, the field element: double,
-// or float in the FP32 form; the sums themselves are FP64)
+// the target's sums, written once
 template <int KIND, typename F>
-__device__ __forceinline__ void store_target(const Acc &A, F *__restrict__ out_s, F *__restrict__ out_t, uint32_t t,
-                                             int64_t n3v) {
+__device__ __forceinline__ void store_target(const Acc<F> &A, F *__restrict__ out_s, F *__restrict__ out_t,
+                                             uint32_t t, int64_t n3v) {
     using K = Kind<KIND>;
     if (K::T00) out_s[t] = (F)A.s00;
     if (K::TIJ) {
@@ -315,8 +322,8 @@ __device__ __forceinline__ void store_target(const Acc &A, F *__restrict__ out_s
     }
 
 // the particles 0 .. m-1 of the records, in order, onto one partial sum
-template <int KIND, int DX, int DY, int DZ, typename ST>
-__device__ __forceinline__ void add_records(Partial &P, const ST &st, const Target &T, uint32_t m) {
+template <int KIND, int DX, int DY, int DZ, typename ST, typename R>
+__device__ __forceinline__ void add_records(Partial<R> &P, const ST &st, const Target<R> &T, uint32_t m) {
 #pragma unroll 1
     for (uint32_t i = 0; i < m; i++) add_particle<KIND>(P, st.get((int)i), T, DX, DY, DZ);
 }
@@ -330,7 +337,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const F *__restrict__ phi, F *__restrict__ out_s, F *__restrict__ out_t, int N,
     int ntx, int nty, int ntz, double a, double mass_s, double mass_t) {
     using K = Kind<KIND>;
-    __shared__ Stage<KIND> st;
+    __shared__ Stage<KIND, F> st;
     __shared__ uint32_t s_off[NSRC + 1];
     __shared__ uint32_t s_gbeg[NSRC];
     __shared__ uint32_t s_cnt[NSRC];
@@ -358,14 +365,15 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
     const bool active = lx < nx && ly < ny && lz < nz;
     const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
-    Target T;
-    T.ph = T.plq01 = T.plq02 = T.plq12 = 0.;
-    T.lnk0 = T.lnk1 = T.lnk2 = 1.;
-    Acc A;
-    A.s00 = A.d0 = A.d1 = A.d2 = A.o01 = A.o02 = A.o12 = A.v0 = A.v1 = A.v2 = 0.;
+    Target<F> T;
+    T.ph = T.plq01 = T.plq02 = T.plq12 = (F)0;
+    T.lnk0 = T.lnk1 = T.lnk2 = (F)1;
+    Acc<F> A;
+    A.s00 = A.d0 = A.d1 = A.d2 = A.o01 = A.o02 = A.o12 = A.v0 = A.v1 = A.v2 = (F)0;
+    const F mass_sf = (F)mass_s;
     const uint32_t t = active ? L(N, x, y, z) : 0u;
     if (active) {
-        double phx = 0., phy = 0., phz = 0., phxy = 0., phxz = 0., phyz = 0.;
+        F phx = 0, phy = 0, phz = 0, phxy = 0, phxz = 0, phyz = 0;
         if (USE_PHI) {
             const int xp = x == N - 1 ? 0 : x + 1, yp = y == N - 1 ? 0 : y + 1, zp = z == N - 1 ? 0 : z + 1;
             T.ph = phi[t];
@@ -380,9 +388,9 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
         T.plq01 = T.ph + phy + phx + phxy;
         T.plq02 = T.ph + phz + phx + phxz;
         T.plq12 = T.ph + phz + phy + phyz;
-        T.lnk0 = 1. + T.ph + phx;
-        T.lnk1 = 1. + T.ph + phy;
-        T.lnk2 = 1. + T.ph + phz;
+        T.lnk0 = (F)1 + T.ph + phx;
+        T.lnk1 = (F)1 + T.ph + phy;
+        T.lnk2 = (F)1 + T.ph + phz;
         if (ACCUM) {
             if (K::T00) A.s00 = out_s[t];
             if (K::TIJ) {
@@ -475,14 +483,14 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             __syncthreads();
             if (active) {
                 const int cc = (lz * SY + ly) * SX + lx;  // source (dx, dy, dz) = (1, 1, 1)
-                add_source_cell<KIND, 1, 1, 1, false>(A, st, T, cc, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 1, 1, false>(A, st, T, cc + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 0, 1, false>(A, st, T, cc + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 0, 1, false>(A, st, T, cc + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 1, 0, false>(A, st, T, cc + SX * SY, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 1, 0, false>(A, st, T, cc + SX * SY + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 0, 0, false>(A, st, T, cc + SX * SY + SX, 0, NSRC, 0u, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 0, 0, false>(A, st, T, cc + SX * SY + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 1, 1, false>(A, st, T, cc, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 0, 1, 1, false>(A, st, T, cc + 1, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 1, 0, 1, false>(A, st, T, cc + SX, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 0, 0, 1, false>(A, st, T, cc + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 1, 1, 0, false>(A, st, T, cc + SX * SY, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 0, 1, 0, false>(A, st, T, cc + SX * SY + 1, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 1, 0, 0, false>(A, st, T, cc + SX * SY + SX, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 0, 0, 0, false>(A, st, T, cc + SX * SY + SX + 1, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
                 store_target<KIND>(A, out_s, out_t, t, n3v);
             }
             return;
@@ -611,15 +619,15 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                 // fixed order (dz, dy, dx) = 111, 110 .. 000 reversed per
                 // axis, i.e. site order, with compile-time offsets
                 const int cc = (lz * nsy + ly) * nsx + lx;  // source (dx, dy, dz) = (1, 1, 1)
-                add_source_cell<KIND, 1, 1, 1>(A, st, T, cc, cb, ce, base, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 1, 1>(A, st, T, cc + 1, cb, ce, base, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 0, 1>(A, st, T, cc + nsx, cb, ce, base, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 0, 1>(A, st, T, cc + nsx + 1, cb, ce, base, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 1, 0>(A, st, T, cc + nsx * nsy, cb, ce, base, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 0, 1, 0>(A, st, T, cc + nsx * nsy + 1, cb, ce, base, s_off, s_cnt, mass_s);
-                add_source_cell<KIND, 1, 0, 0>(A, st, T, cc + nsx * nsy + nsx, cb, ce, base, s_off, s_cnt, mass_s);
+                add_source_cell<KIND, 1, 1, 1>(A, st, T, cc, cb, ce, base, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 0, 1, 1>(A, st, T, cc + 1, cb, ce, base, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 1, 0, 1>(A, st, T, cc + nsx, cb, ce, base, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 0, 0, 1>(A, st, T, cc + nsx + 1, cb, ce, base, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 1, 1, 0>(A, st, T, cc + nsx * nsy, cb, ce, base, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 0, 1, 0>(A, st, T, cc + nsx * nsy + 1, cb, ce, base, s_off, s_cnt, mass_sf);
+                add_source_cell<KIND, 1, 0, 0>(A, st, T, cc + nsx * nsy + nsx, cb, ce, base, s_off, s_cnt, mass_sf);
                 add_source_cell<KIND, 0, 0, 0>(A, st, T, cc + nsx * nsy + nsx + 1, cb, ce, base, s_off, s_cnt,
-                                               mass_s);
+                                               mass_sf);
             } else if (active) {
                 // the 8 sources in ascending site order: along each axis the
                 // lower neighbour (offset 1) comes first, except at
@@ -630,7 +638,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                     const int dxx = x > 0 ? 1 - ix : ix, dy = y > 0 ? 1 - iy : iy, dz = z > 0 ? 1 - iz : iz;
                     const int c = (order_pos(lz + 1 - dz, z0, nsz) * nsy + order_pos(ly + 1 - dy, y0, nsy)) * nsx +
                                   order_pos(lx + 1 - dxx, x0, nsx);
-#define GEV_SRC(X, Y, Z) add_source_cell<KIND, X, Y, Z>(A, st, T, c, cb, ce, base, s_off, s_cnt, mass_s)
+#define GEV_SRC(X, Y, Z) add_source_cell<KIND, X, Y, Z>(A, st, T, c, cb, ce, base, s_off, s_cnt, mass_sf)
                     GEV_BY_OFFSET(dxx | (dy << 1) | (dz << 2), GEV_SRC)
 #undef GEV_SRC
                 }
@@ -656,7 +664,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             const double rx = wrapi(x0 - 1 + order_slot(px, x0, nsx), N) * dx;
             const double ry = wrapi(y0 - 1 + order_slot(py, y0, nsy), N) * dx;
             const double rz = wrapi(z0 - 1 + order_slot(pz, z0, nsz), N) * dx;
-            Partial P;
+            Partial<F> P;
             P.clear();
             for (uint32_t s0 = 0; s0 < n; s0 += (uint32_t)CAP) {
                 const uint32_t m = min((uint32_t)CAP, n - s0);
@@ -671,7 +679,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
 #undef GEV_REC
                 __syncthreads();
             }
-#define GEV_ADD(X, Y, Z) add_source<KIND>(A, P, T, X, Y, Z, mass_s)
+#define GEV_ADD(X, Y, Z) add_source<KIND>(A, P, T, X, Y, Z, mass_sf)
             if (d >= 0) {
                 GEV_BY_OFFSET(d, GEV_ADD)
             }

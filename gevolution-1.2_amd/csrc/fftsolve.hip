@@ -43,15 +43,25 @@ template <> struct ZSolve<ZS_VECTOR> { static constexpr int NIN = 3, NOUT = 3; }
 // components of its points in registers through the solve, so the tensor
 // solve (6 inputs + the carried B) takes fewer points per thread, and with
 // them more threads, to keep its register count -- and the waves per SIMD
-// that hide the column loads -- in check.  GEV_ZS_COLS / GEV_ZS_PTS_TENSOR
-// override the defaults.
+// that hide the column loads -- in check.  The FP32 form (C = float2) has
+// its own defaults.  GEV_ZS_COLS / GEV_ZS_PTS_TENSOR (and the _F32 pair)
+// override them.
 #ifndef GEV_ZS_COLS
 #define GEV_ZS_COLS 4
 #endif
 #ifndef GEV_ZS_PTS_TENSOR
 #define GEV_ZS_PTS_TENSOR 4
 #endif
-template <int N> constexpr int zs_cols() { return N >= 768 ? 2 : GEV_ZS_COLS; }
+#ifndef GEV_ZS_COLS_F32
+#define GEV_ZS_COLS_F32 4
+#endif
+#ifndef GEV_ZS_PTS_TENSOR_F32
+#define GEV_ZS_PTS_TENSOR_F32 4
+#endif
+template <typename C> constexpr bool is_f32() { return sizeof(C) == 8; }
+template <int N, typename C> constexpr int zs_cols() {
+    return N >= 768 ? 2 : (is_f32<C>() ? GEV_ZS_COLS_F32 : GEV_ZS_COLS);
+}
 // The tensor solve prefetches the carried B into registers during the last
 // input transform where the registers allow it; at N = 384 and above 512 the
 // prefetch would spill (compiler resource report), so B is read at the solve.
@@ -60,9 +70,12 @@ template <int N> constexpr int zs_cols() { return N >= 768 ? 2 : GEV_ZS_COLS; }
 #endif
 constexpr int ZS_AHEAD = GEV_ZS_AHEAD;
 template <int N> constexpr bool zs_prefetch_b() { return ZS_AHEAD == 1 && N <= 512 && N != 384; }
-template <int KIND> constexpr int zs_pts() { return KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR ? GEV_ZS_PTS_TENSOR : 4; }
-template <int N, int KIND> constexpr int zs_threads() {
-    int t = N * zs_cols<N>() / zs_pts<KIND>();
+template <int KIND, typename C> constexpr int zs_pts() {
+    return KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR ? (is_f32<C>() ? GEV_ZS_PTS_TENSOR_F32 : GEV_ZS_PTS_TENSOR)
+                                                         : 4;
+}
+template <int N, int KIND, typename C> constexpr int zs_threads() {
+    int t = N * zs_cols<N, C>() / zs_pts<KIND, C>();
     if (t > 1024) t = 1024;
     if (t < 64) t = 64;
     return (t + 63) / 64 * 64;
@@ -87,9 +100,9 @@ template <typename C> struct ZArgsT {
 using ZArgs = ZArgsT<double2>;
 
 template <int N, int KIND, typename C>
-__global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgsT<C> A) {
-    constexpr int W = zs_cols<N>();
-    constexpr int NT = zs_threads<N, KIND>();
+__global__ __launch_bounds__((zs_threads<N, KIND, C>())) void k_zsolve(ZArgsT<C> A) {
+    constexpr int W = zs_cols<N, C>();
+    constexpr int NT = zs_threads<N, KIND, C>();
     constexpr int NIN = ZSolve<KIND>::NIN, NOUT = ZSolve<KIND>::NOUT;
     constexpr int PTS = (N * W + NT - 1) / NT;
     constexpr bool EXACT = PTS * NT == N * W;
@@ -167,8 +180,9 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgsT<C> A)
         }
     }
 
-    // the solve, per k point (kx, ky, kz = element index)
-    double2 O[NOUT][PTS];
+    // the solve, per k point (kx, ky, kz = element index), in FP64; the
+    // outputs are kept in the storage type (k = 0 of output 0 also in FP64)
+    C O[NOUT][PTS];
     const double gx = A.g[ok ? kx : 0];
     const double2 kxs = A.ks[ok ? kx : 0];
     const double gy = A.g[ky];
@@ -182,13 +196,14 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgsT<C> A)
         const double2 kzs = A.ks[live ? kz : 0];
         const bool zero = kx == 0 && ky == 0 && kz == 0;
         const int64_t at = base + (int64_t)kz * es + w;
+        double2 o0;
         if constexpr (KIND == ZS_POISSON) {
-            O[0][j] = poisson_point(d2(S[0][j]), zero, zero ? 0. : gx + gy + gz, A.c0, A.c1);
+            o0 = poisson_point(d2(S[0][j]), zero, zero ? 0. : gx + gy + gz, A.c0, A.c1);
         } else if constexpr (KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR) {
             double2 Sp[6];
 #pragma unroll
             for (int c = 0; c < 6; c++) Sp[c] = d2(S[c][j]);
-            O[0][j] = zero ? c2(0., 0.) : project_scalar(Sp, gx, gy, gz, kxs, kys, kzs, N);
+            o0 = zero ? c2(0., 0.) : project_scalar(Sp, gx, gy, gz, kxs, kys, kzs, N);
             if constexpr (KIND == ZS_TENSOR) {
                 double2 B[3] = {c2(0., 0.), c2(0., 0.), c2(0., 0.)};
                 if (live) {
@@ -209,7 +224,7 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgsT<C> A)
                     A.bift[2 * A.nkv + at] = from_d2<C>(B[2]);
                 }
 #pragma unroll
-                for (int c = 0; c < 3; c++) O[1 + c][j] = B[c];
+                for (int c = 0; c < 3; c++) O[1 + c][j] = from_d2<C>(B[c]);
             }
         } else {
             double2 Sv[3] = {d2(S[0][j]), d2(S[1][j]), d2(S[2][j])};
@@ -223,11 +238,13 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgsT<C> A)
                 A.bift[A.nkv + at] = from_d2<C>(Sv[1]);
                 A.bift[2 * A.nkv + at] = from_d2<C>(Sv[2]);
             }
+            o0 = Sv[0];
 #pragma unroll
-            for (int c = 0; c < 3; c++) O[c][j] = Sv[c];
+            for (int c = 1; c < 3; c++) O[c][j] = from_d2<C>(Sv[c]);
         }
-        if (live && A.kstore) A.kstore[at] = from_d2<C>(O[0][j]);
-        if (zero && A.k0) A.k0[0] = O[0][j];
+        O[0][j] = from_d2<C>(o0);
+        if (live && A.kstore) A.kstore[at] = O[0][j];
+        if (zero && A.k0) A.k0[0] = o0;
     }
 
     // inverse z transform of every output (conjugate, forward stages, conjugate)
@@ -236,7 +253,7 @@ __global__ __launch_bounds__((zs_threads<N, KIND>())) void k_zsolve(ZArgsT<C> A)
 #pragma unroll
         for (int j = 0; j < PTS; j++) {
             const int idx = threadIdx.x + j * NT;
-            if (EXACT || idx < N * W) buf[idx] = from_d2<C>(cconj(O[o][j]));
+            if (EXACT || idx < N * W) buf[idx] = cconj(O[o][j]);
         }
         __syncthreads();
         stages<C, N, W, true, 1, NT>(buf, tw);
@@ -392,9 +409,9 @@ template <int NC> int launch_x_nc(gev_ctx *c, XArgs A) {
 // ---------------------------------------------------------------- last pass
 
 template <int N, int KIND, typename C> int launch_n(gev_ctx *c, const ZArgsT<C> &A) {
-    constexpr int W = zs_cols<N>();
+    constexpr int W = zs_cols<N, C>();
     const int H = N / 2 + 1;
-    hipLaunchKernelGGL((k_zsolve<N, KIND, C>), dim3((H + W - 1) / W, N), dim3(zs_threads<N, KIND>()), 0, c->stream,
+    hipLaunchKernelGGL((k_zsolve<N, KIND, C>), dim3((H + W - 1) / W, N), dim3(zs_threads<N, KIND, C>()), 0, c->stream,
                        A);
     return gev::check_hip(c, hipGetLastError(), "fused z solve launch");
 }

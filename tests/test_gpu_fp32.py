@@ -2,16 +2,16 @@
 plain FP64 form, which the oracle pins.
 
 The FP32 form stores phi, chi, source, Sij, Bi and the k-space fields as
-float and runs the FFT butterflies in float; particles, the background, the
-deposit sums, the stencils' arithmetic and the k-space solves (denominators,
-zero mode) stay FP64.  Its results therefore differ from the FP64 form by
-float rounding of the stored fields, and the bars are stated per quantity:
+float and runs the FFT butterflies and the deposit's per-particle sums in
+float; particles, the background, the stencils' arithmetic and the k-space
+solves (denominators, zero mode) stay FP64.  Its results therefore differ
+from the FP64 form by float rounding, and the bars are stated per quantity:
   * one FFT round trip and a forward transform: max error <= 2e-6 x max |ref|
     (about 17 ulp of float at the magnitudes involved, log2 N stages);
   * whole cycles from the same state: phi, chi within 1e-4 relative (max
-    norm), Bi within 1e-3 (a small field built from differences), momenta
-    within 1e-5 of their change over the run, positions within 1e-9
-    absolute (box units);
+    norm; phi(k=0), the mean of phi, within 1e-4 of max |phi|), Bi within
+    1e-3 (a small field built from differences), momenta within 1e-5 of
+    their change over the run, positions within 1e-9 absolute (box units);
   * the matter power spectrum (T00 with phi, FFT, extractPowerSpectrum) of
     generated LCDM initial conditions evolved in both forms: within 1e-4
     relative in every bin below k_Nyq/2;
@@ -117,6 +117,7 @@ def run_forms(gev, N, boxsize, ncycles, gr=True, elliptic=False, seed=31):
         e = {}
         for name, f in (("phi", gev.PHI), ("chi", gev.CHI), ("Bi", gev.BI)):
             e[name] = rel(b32.download_field(f), b64.download_field(f))
+        phimax = np.max(np.abs(b64.download_field(gev.PHI)))
         p64, q64, i64 = b64.download_particles()
         p32, q32, i32 = b32.download_particles()
         o64, o32 = np.argsort(i64), np.argsort(i32)
@@ -124,7 +125,8 @@ def run_forms(gev, N, boxsize, ncycles, gr=True, elliptic=False, seed=31):
         e["pos_abs"] = np.max(np.abs(p32[o32] - p64[o64]))
         e["dq"] = rel(q32[o32] - q0, q64[o64] - q0)
         e["T00hom"] = abs(s32.T00hom - s64.T00hom) / abs(s64.T00hom) if gr else 0.0
-        e["phi_k0"] = abs(s32.phi_k0 - s64.phi_k0) / max(abs(s64.phi_k0), 1e-300)
+        # phi(k=0), the mean of phi, against the scale of phi
+        e["phi_k0"] = abs(s32.phi_k0 - s64.phi_k0) / phimax
         errs.append(e)
     b64.close()
     b32.close()
@@ -142,7 +144,7 @@ def test_fp32_cycles_track_fp64(gev, N, gr, elliptic):
         assert e["pos_abs"] <= 1e-9, (c, e)
         assert e["dq"] <= 1e-5, (c, e)
         assert e["T00hom"] <= 1e-6, (c, e)
-        assert e["phi_k0"] <= 1e-3, (c, e)  # the zero mode: a small difference of large sums
+        assert e["phi_k0"] <= 1e-4, (c, e)
 
 
 def test_fp32_run_to_run_bit_identical(gev):
