@@ -56,7 +56,7 @@ template <> struct ZSolve<ZS_VECTOR> { static constexpr int NIN = 3, NOUT = 3; }
 #define GEV_ZS_COLS_F32 4
 #endif
 #ifndef GEV_ZS_PTS_TENSOR_F32
-#define GEV_ZS_PTS_TENSOR_F32 4
+#define GEV_ZS_PTS_TENSOR_F32 2
 #endif
 template <typename C> constexpr bool is_f32() { return sizeof(C) == 8; }
 template <int N, typename C> constexpr int zs_cols() {
@@ -65,17 +65,24 @@ template <int N, typename C> constexpr int zs_cols() {
 // The tensor solve prefetches the carried B into registers during the last
 // input transform where the registers allow it; at N = 384 and above 512 the
 // prefetch would spill (compiler resource report), so B is read at the solve.
+// The FP32 form reads B at the solve (measured no slower, fewer registers).
 #ifndef GEV_ZS_AHEAD
 #define GEV_ZS_AHEAD 1
 #endif
 constexpr int ZS_AHEAD = GEV_ZS_AHEAD;
-template <int N> constexpr bool zs_prefetch_b() { return ZS_AHEAD == 1 && N <= 512 && N != 384; }
-template <int KIND, typename C> constexpr int zs_pts() {
-    return KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR ? (is_f32<C>() ? GEV_ZS_PTS_TENSOR_F32 : GEV_ZS_PTS_TENSOR)
-                                                         : 4;
+#ifndef GEV_ZS_PREFETCH_F32
+#define GEV_ZS_PREFETCH_F32 0
+#endif
+template <int N, typename C = double2> constexpr bool zs_prefetch_b() {
+    return ZS_AHEAD == 1 && N <= 512 && N != 384 && (GEV_ZS_PREFETCH_F32 || !is_f32<C>());
+}
+template <int N, int KIND, typename C> constexpr int zs_pts() {
+    if (KIND != ZS_TENSOR && KIND != ZS_TENSOR_SCALAR) return 4;
+    if (!is_f32<C>()) return GEV_ZS_PTS_TENSOR;
+    return N >= 768 ? 4 : GEV_ZS_PTS_TENSOR_F32;  // two columns above 512: 1024 threads would spill
 }
 template <int N, int KIND, typename C> constexpr int zs_threads() {
-    int t = N * zs_cols<N, C>() / zs_pts<KIND, C>();
+    int t = N * zs_cols<N, C>() / zs_pts<N, KIND, C>();
     if (t > 1024) t = 1024;
     if (t < 64) t = 64;
     return (t + 63) / 64 * 64;
@@ -168,7 +175,7 @@ __global__ __launch_bounds__((zs_threads<N, KIND, C>())) void k_zsolve(ZArgsT<C>
         // the following components' loads stay in flight through this
         // component's transform (the stage barriers wait for LDS traffic only)
         if (c + ZS_AHEAD < NIN) load_comp(c + ZS_AHEAD, v[c % ZS_AHEAD]);
-        else if (c + 1 == NIN && KIND == ZS_TENSOR && zs_prefetch_b<N>()) load_bold();
+        else if (c + 1 == NIN && KIND == ZS_TENSOR && zs_prefetch_b<N, C>()) load_bold();
         __syncthreads();
         stages<C, N, W, true, 1, NT>(buf, tw);
         // each thread reads back only the points it wrote, so the next
@@ -199,6 +206,35 @@ __global__ __launch_bounds__((zs_threads<N, KIND, C>())) void k_zsolve(ZArgsT<C>
         double2 o0;
         if constexpr (KIND == ZS_POISSON) {
             o0 = poisson_point(d2(S[0][j]), zero, zero ? 0. : gx + gy + gz, A.c0, A.c1);
+        } else if constexpr ((KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR) && is_f32<C>()) {
+            // FP32 form: FP64 coefficients and denominators, float field combinations
+            float2 Sf[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) Sf[c] = S[c][j];
+            const float2 chi = zero ? make_float2(0.f, 0.f) : project_scalar_f32(Sf, gx, gy, gz, kxs, kys, kzs, N);
+            o0 = make_double2(chi.x, chi.y);
+            if constexpr (KIND == ZS_TENSOR) {
+                float2 B[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+                if (live) {
+                    if (!zero) {
+                        if constexpr (zs_prefetch_b<N, C>()) {
+                            B[0] = Bold[0][j];
+                            B[1] = Bold[KIND == ZS_TENSOR ? 1 : 0][j];
+                            B[2] = Bold[KIND == ZS_TENSOR ? 2 : 0][j];
+                        } else {
+                            B[0] = A.bift[at];
+                            B[1] = A.bift[A.nkv + at];
+                            B[2] = A.bift[2 * A.nkv + at];
+                        }
+                        evolve_vector_f32(Sf, B, gx, gy, gz, kxs, kys, kzs, A.c0);
+                    }
+                    A.bift[at] = B[0];
+                    A.bift[A.nkv + at] = B[1];
+                    A.bift[2 * A.nkv + at] = B[2];
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++) O[1 + c][j] = B[c];
+            }
         } else if constexpr (KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR) {
             double2 Sp[6];
 #pragma unroll
@@ -208,7 +244,7 @@ __global__ __launch_bounds__((zs_threads<N, KIND, C>())) void k_zsolve(ZArgsT<C>
                 double2 B[3] = {c2(0., 0.), c2(0., 0.), c2(0., 0.)};
                 if (live) {
                     if (!zero) {
-                        if constexpr (zs_prefetch_b<N>()) {
+                        if constexpr (zs_prefetch_b<N, C>()) {
                             B[0] = d2(Bold[0][j]);
                             B[1] = d2(Bold[KIND == ZS_TENSOR ? 1 : 0][j]);
                             B[2] = d2(Bold[KIND == ZS_TENSOR ? 2 : 0][j]);

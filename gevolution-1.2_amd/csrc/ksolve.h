@@ -97,4 +97,64 @@ __device__ __forceinline__ void project_vector_point(double2 S[3], double2 k0, d
     S[2] = cdivr(cmulr(cmulr(csub(S2, cmul(cconj(k2v), tmp)), 4.), coeff), k2 + modif);
 }
 
+// ---- the FP32 form's tensor solves: the k-dependent coefficients and the
+// denominators (k -> 0) are evaluated in FP64 and rounded once; only the
+// combination linear in the float field values runs in float.  Same terms
+// as project_scalar / evolve_vector above, regrouped by source component.
+
+__device__ __forceinline__ float2 f2(double2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ float2 fmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 faddmul(float2 acc, float2 c, float2 s) {
+    const float2 p = fmulc(c, s);
+    return make_float2(acc.x + p.x, acc.y + p.y);
+}
+__device__ __forceinline__ float2 faddmulr(float2 acc, float c, float2 s) {
+    return make_float2(acc.x + c * s.x, acc.y + c * s.y);
+}
+
+// projectFTscalar body, gevolution.hpp:251-260 (k != 0)
+__device__ __forceinline__ float2 project_scalar_f32(const float2 S[6], double g0, double g1, double g2, double2 k0,
+                                                     double2 k1, double2 k2, int N) {
+    const double inv = 1. / (2. * (g0 + g1 + g2) * (g0 + g1 + g2) * (double)N);
+    float2 num = make_float2(0.f, 0.f);
+    num = faddmulr(num, (float)((g1 + g2 - 2. * g0) * inv), S[0]);
+    num = faddmulr(num, (float)((g0 + g2 - 2. * g1) * inv), S[3]);
+    num = faddmulr(num, (float)((g0 + g1 - 2. * g2) * inv), S[5]);
+    num = faddmul(num, f2(rmul(-inv, cmul(rmul(6., k0), k1))), S[1]);
+    num = faddmul(num, f2(rmul(-inv, cmul(rmul(6., k0), k2))), S[2]);
+    num = faddmul(num, f2(rmul(-inv, cmul(rmul(6., k1), k2))), S[4]);
+    return num;
+}
+
+// evolveFTvector body, gevolution.hpp:314-325 (k != 0); adds onto B[3].  With
+// p = -2 a^2 dtau / k^4 and the component axes (i, j, l) = (0, 1, 2), (1, 0, 2)
+// and (2, 0, 1):  B_i += i p [ conj(k_i) ((g_j + g_l) S_ii - g_j S_jj - g_l S_ll
+//                              - 2 k_j k_l S_jl) + (g_j + g_l - g_i)(k_j S_ij + k_l S_il) ]
+__device__ __forceinline__ void evolve_vector_f32(const float2 S[6], float2 B[3], double g0, double g1, double g2,
+                                                  double2 k0, double2 k1, double2 k2, double a2dtau) {
+    double k4 = g0 + g1 + g2;
+    k4 *= k4;
+    const double p = -2. * a2dtau / k4;
+    const double g[3] = {g0, g1, g2};
+    const double2 kk[3] = {k0, k1, k2};
+    // symmetric index of (a, b) in S: 00 01 02 11 12 22
+    constexpr int SI[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int j = i == 0 ? 1 : 0, l = i == 2 ? 1 : 2;
+        const double2 ip_ck = cmul(c2(0., p), cconj(kk[i]));       // i p conj(k_i)
+        const double2 ipg = c2(0., p * (g[j] + g[l] - g[i]));       // i p (g_j + g_l - g_i)
+        float2 acc = B[i];
+        acc = faddmul(acc, f2(rmul(g[j] + g[l], ip_ck)), S[SI[i][i]]);
+        acc = faddmul(acc, f2(rmul(-g[j], ip_ck)), S[SI[j][j]]);
+        acc = faddmul(acc, f2(rmul(-g[l], ip_ck)), S[SI[l][l]]);
+        acc = faddmul(acc, f2(cmul(ip_ck, rmul(-2., cmul(kk[j], kk[l])))), S[SI[j][l]]);
+        acc = faddmul(acc, f2(cmul(ipg, kk[j])), S[SI[i][j]]);
+        acc = faddmul(acc, f2(cmul(ipg, kk[l])), S[SI[i][l]]);
+        B[i] = acc;
+    }
+}
+
 }  // namespace gev_ks

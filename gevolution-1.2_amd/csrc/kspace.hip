@@ -57,11 +57,36 @@ __global__ __launch_bounds__(TPB) void k_sij_solve(const C *__restrict__ SijFT,
     KIdx k = kindex(i, P, N);
     if (!k.valid) return;
     const bool zero = k.kx == 0 && k.ky == 0 && k.kz == 0;
+    const double g0 = g[k.kx], g1 = g[k.ky], g2 = g[k.kz];
+    const double2 k0 = ks[k.kx], k1 = ks[k.ky], k2 = ks[k.kz];
+    if constexpr (sizeof(C) == 8) {
+        // FP32 form: the bodies the fused z pass uses (FP64 coefficients and
+        // denominators, float field combinations)
+        float2 S[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) S[c] = SijFT[c * nkv + i];
+        if (MODE & 1) {
+            if (zero) {
+                chiFT[i] = make_float2(0.f, 0.f);
+            } else {
+                const float2 r = project_scalar_f32(S, g0, g1, g2, k0, k1, k2, N);
+                chiFT[i] = add ? make_float2(chiFT[i].x + r.x, chiFT[i].y + r.y) : r;
+            }
+        }
+        if (MODE & 2) {
+            if (zero) {
+                for (int c = 0; c < 3; c++) BiFT[c * nkv + i] = make_float2(0.f, 0.f);
+            } else {
+                float2 B[3] = {BiFT[i], BiFT[nkv + i], BiFT[2 * nkv + i]};
+                evolve_vector_f32(S, B, g0, g1, g2, k0, k1, k2, a2dtau);
+                for (int c = 0; c < 3; c++) BiFT[c * nkv + i] = B[c];
+            }
+        }
+        return;
+    }
     double2 S[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) S[c] = d2(SijFT[c * nkv + i]);
-    const double g0 = g[k.kx], g1 = g[k.ky], g2 = g[k.kz];
-    const double2 k0 = ks[k.kx], k1 = ks[k.ky], k2 = ks[k.kz];
     if (MODE & 1) {
         if (zero) {
             chiFT[i] = from_d2<C>(c2(0., 0.));
