@@ -224,6 +224,9 @@ template <int LO, typename F> struct FieldView {
     __device__ __forceinline__ double operator()(int dx, int dy, int dz) const {
         return (double)f[base + dx + Region<LO>::SX * (dy + Region<LO>::SY * dz)];
     }
+    __device__ __forceinline__ F raw(int dx, int dy, int dz) const {
+        return f[base + dx + Region<LO>::SX * (dy + Region<LO>::SY * dz)];
+    }
 };
 template <int LO, typename F>
 __device__ __forceinline__ FieldView<LO, F> view(const F *f, const int c[3], int x0, int y0, int z0) {
@@ -416,6 +419,138 @@ __device__ __forceinline__ void update_pos(double pos[3], const double q[3], con
         b[2] += B2(1, 1, 0) * rd[0] * rd[1];
 #pragma unroll
         for (int l = 0; l < 3; l++) pos[l] += dtau * (v[l] + b[l] / params[1]);
+    } else {
+#pragma unroll
+        for (int l = 0; l < 3; l++) pos[l] += dtau * v[l];
+    }
+}
+
+// ---- The FP32 form's gathers.  The CIC gradient and interpolation stencils
+// over the float fields run in float (the weights are the position's
+// fractional parts rounded once); the momentum and position updates
+// themselves, e = sqrt(q^2 + a^2) and every factor involving q, stay FP64.
+// Same terms as update_q / update_q_newton / update_pos above.
+
+// trilinear gradient of a float field at fractional position (r0, r1, r2)
+template <int LO>
+__device__ __forceinline__ void grad_f32(FieldView<LO, float> f, float r0, float r1, float r2, float g[3]) {
+    const float s0 = 1.f - r0, s1 = 1.f - r1, s2 = 1.f - r2;
+    g[0] = s1 * s2 * (f.raw(1, 0, 0) - f.raw(0, 0, 0)) + r1 * s2 * (f.raw(1, 1, 0) - f.raw(0, 1, 0)) +
+           s1 * r2 * (f.raw(1, 0, 1) - f.raw(0, 0, 1)) + r1 * r2 * (f.raw(1, 1, 1) - f.raw(0, 1, 1));
+    g[1] = s0 * s2 * (f.raw(0, 1, 0) - f.raw(0, 0, 0)) + r0 * s2 * (f.raw(1, 1, 0) - f.raw(1, 0, 0)) +
+           s0 * r2 * (f.raw(0, 1, 1) - f.raw(0, 0, 1)) + r0 * r2 * (f.raw(1, 1, 1) - f.raw(1, 0, 1));
+    g[2] = s0 * s1 * (f.raw(0, 0, 1) - f.raw(0, 0, 0)) + r0 * s1 * (f.raw(1, 0, 1) - f.raw(1, 0, 0)) +
+           s0 * r1 * (f.raw(0, 1, 1) - f.raw(0, 1, 0)) + r0 * r1 * (f.raw(1, 1, 1) - f.raw(1, 1, 0));
+}
+
+// trilinear interpolation of a float field
+__device__ __forceinline__ float interp_f32(ViewS<float> f, float r0, float r1, float r2) {
+    const float s0 = 1.f - r0, s1 = 1.f - r1, s2 = 1.f - r2;
+    return s2 * (s1 * (s0 * f.raw(0, 0, 0) + r0 * f.raw(1, 0, 0)) + r1 * (s0 * f.raw(0, 1, 0) + r0 * f.raw(1, 1, 0))) +
+           r2 * (s1 * (s0 * f.raw(0, 0, 1) + r0 * f.raw(1, 0, 1)) + r1 * (s0 * f.raw(0, 1, 1) + r0 * f.raw(1, 1, 1)));
+}
+
+// update_q, gevolution.hpp:570-678, FP32 form
+template <bool POW2>
+__device__ __forceinline__ double update_q(double q[3], const double rd[3], ViewS<float> phi, ViewS<float> chi,
+                                           View0<float> B0, View1<float> B1, View2<float> B2, int nfield,
+                                           const double params[2], double dtau, double dx) {
+    const float r0 = (float)rd[0], r1 = (float)rd[1], r2 = (float)rd[2];
+    const float s0 = 1.f - r0, s1 = 1.f - r1, s2 = 1.f - r2;
+    double v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    double e2 = v2 + params[0] * params[0];
+    float g[3];
+    grad_f32(phi, r0, r1, r2, g);
+    const double fac = (v2 + e2) / e2;
+    double gradphi[3] = {g[0] * fac, g[1] * fac, g[2] * fac};
+    GEV_LDS_PHASE();
+    if (nfield >= 2) {
+        grad_f32(chi, r0, r1, r2, g);
+        gradphi[0] -= g[0];
+        gradphi[1] -= g[1];
+        gradphi[2] -= g[2];
+    }
+    e2 = sqrt(e2);
+    GEV_LDS_PHASE();
+    if (nfield >= 3) {
+        // pgradB_i = sum_j A_ij q_j with the A_ij float stencils of B
+        const float a01 = s2 * (B1.raw(1, 0, 0) - B1.raw(0, 0, 0)) + r2 * (B1.raw(1, 0, 1) - B1.raw(0, 0, 1));
+        const float a02 = s1 * (B2.raw(1, 0, 0) - B2.raw(0, 0, 0)) + r1 * (B2.raw(1, 1, 0) - B2.raw(0, 1, 0));
+        const float c0 = r0 - 1.f, c1 = 1.f - 2.f * r0;
+        const float a00 = s1 * s2 * (c0 * B0.raw(-1, 0, 0) + c1 * B0.raw(0, 0, 0) + r0 * B0.raw(1, 0, 0)) +
+                          r1 * s2 * (c0 * B0.raw(-1, 1, 0) + c1 * B0.raw(0, 1, 0) + r0 * B0.raw(1, 1, 0)) +
+                          s1 * r2 * (c0 * B0.raw(-1, 0, 1) + c1 * B0.raw(0, 0, 1) + r0 * B0.raw(1, 0, 1)) +
+                          r1 * r2 * (c0 * B0.raw(-1, 1, 1) + c1 * B0.raw(0, 1, 1) + r0 * B0.raw(1, 1, 1));
+        GEV_LDS_PHASE();
+        const float a12 = s0 * (B2.raw(0, 1, 0) - B2.raw(0, 0, 0)) + r0 * (B2.raw(1, 1, 0) - B2.raw(1, 0, 0));
+        const float a10 = s2 * (B0.raw(0, 1, 0) - B0.raw(0, 0, 0)) + r2 * (B0.raw(0, 1, 1) - B0.raw(0, 0, 1));
+        const float d0 = r1 - 1.f, d1 = 1.f - 2.f * r1;
+        const float a11 = s0 * s2 * (d0 * B1.raw(0, -1, 0) + d1 * B1.raw(0, 0, 0) + r1 * B1.raw(0, 1, 0)) +
+                          r0 * s2 * (d0 * B1.raw(1, -1, 0) + d1 * B1.raw(1, 0, 0) + r1 * B1.raw(1, 1, 0)) +
+                          s0 * r2 * (d0 * B1.raw(0, -1, 1) + d1 * B1.raw(0, 0, 1) + r1 * B1.raw(0, 1, 1)) +
+                          r0 * r2 * (d0 * B1.raw(1, -1, 1) + d1 * B1.raw(1, 0, 1) + r1 * B1.raw(1, 1, 1));
+        GEV_LDS_PHASE();
+        const float a20 = s1 * (B0.raw(0, 0, 1) - B0.raw(0, 0, 0)) + r1 * (B0.raw(0, 1, 1) - B0.raw(0, 1, 0));
+        const float a21 = s0 * (B1.raw(0, 0, 1) - B1.raw(0, 0, 0)) + r0 * (B1.raw(1, 0, 1) - B1.raw(1, 0, 0));
+        const float e0 = r2 - 1.f, e1 = 1.f - 2.f * r2;
+        const float a22 = s0 * s1 * (e0 * B2.raw(0, 0, -1) + e1 * B2.raw(0, 0, 0) + r2 * B2.raw(0, 0, 1)) +
+                          r0 * s1 * (e0 * B2.raw(1, 0, -1) + e1 * B2.raw(1, 0, 0) + r2 * B2.raw(1, 0, 1)) +
+                          s0 * r1 * (e0 * B2.raw(0, 1, -1) + e1 * B2.raw(0, 1, 0) + r2 * B2.raw(0, 1, 1)) +
+                          r0 * r1 * (e0 * B2.raw(1, 1, -1) + e1 * B2.raw(1, 1, 0) + r2 * B2.raw(1, 1, 1));
+        GEV_LDS_PHASE();
+        const double pb0 = a01 * q[1] + a02 * q[2] + a00 * q[0];
+        const double pb1 = a12 * q[2] + a10 * q[0] + a11 * q[1];
+        const double pb2 = a20 * q[0] + a21 * q[1] + a22 * q[2];
+        gradphi[0] += pb0 / params[1] / e2;
+        gradphi[1] += pb1 / params[1] / e2;
+        gradphi[2] += pb2 / params[1] / e2;
+    }
+    v2 = 0.;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        q[i] -= over_dx<POW2>(dtau * e2 * gradphi[i], dx, 1.0 / dx);
+        v2 += q[i] * q[i];
+    }
+    return v2 / params[0] / params[0];
+}
+
+// update_q_Newton, gevolution.hpp:709-776, FP32 form
+template <bool POW2>
+__device__ __forceinline__ double update_q_newton(double q[3], const double rd[3], ViewS<float> psi,
+                                                  ViewS<float> chi, int nfield, const double params[2],
+                                                  double dtau, double dx) {
+    const float r0 = (float)rd[0], r1 = (float)rd[1], r2 = (float)rd[2];
+    float g[3], c[3] = {0.f, 0.f, 0.f};
+    grad_f32(psi, r0, r1, r2, g);
+    if (nfield >= 2) grad_f32(chi, r0, r1, r2, c);
+    double v2 = 0.;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        q[i] -= over_dx<POW2>(dtau * params[0] * (double)(g[i] - c[i]), dx, 1.0 / dx);
+        v2 += q[i] * q[i];
+    }
+    return v2 / params[0] / params[0];
+}
+
+// update_pos, gevolution.hpp:810-871, FP32 form
+__device__ __forceinline__ void update_pos(double pos[3], const double q[3], const double rd[3], ViewS<float> phi,
+                                           ViewS<float> chi, View0<float> B0, View1<float> B1, View2<float> B2,
+                                           int nfield, const double params[2], double dtau) {
+    const float r0 = (float)rd[0], r1 = (float)rd[1], r2 = (float)rd[2];
+    const float s0 = 1.f - r0, s1 = 1.f - r1, s2 = 1.f - r2;
+    double v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const double e2 = v2 + params[0] * params[0];
+    const double ph = nfield >= 1 ? (double)interp_f32(phi, r0, r1, r2) : 0.;
+    const double ch = nfield >= 2 ? (double)interp_f32(chi, r0, r1, r2) : 0.;
+    v2 = (1. + (3. - v2 / e2) * ph - ch) / sqrt(e2);
+    double v[3] = {q[0] * v2, q[1] * v2, q[2] * v2};
+    if (nfield >= 3) {
+        const float b0 = s2 * (s1 * B0.raw(0, 0, 0) + r1 * B0.raw(0, 1, 0)) + r2 * (s1 * B0.raw(0, 0, 1) + r1 * B0.raw(0, 1, 1));
+        const float b1 = s2 * (s0 * B1.raw(0, 0, 0) + r0 * B1.raw(1, 0, 0)) + r2 * (s0 * B1.raw(0, 0, 1) + r0 * B1.raw(1, 0, 1));
+        const float b2 = s1 * (s0 * B2.raw(0, 0, 0) + r0 * B2.raw(1, 0, 0)) + r1 * (s0 * B2.raw(0, 1, 0) + r0 * B2.raw(1, 1, 0));
+        pos[0] += dtau * (v[0] + b0 / params[1]);
+        pos[1] += dtau * (v[1] + b1 / params[1]);
+        pos[2] += dtau * (v[2] + b2 / params[1]);
     } else {
 #pragma unroll
         for (int l = 0; l < 3; l++) pos[l] += dtau * v[l];
