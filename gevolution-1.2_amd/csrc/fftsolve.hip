@@ -53,14 +53,24 @@ template <> struct ZSolve<ZS_VECTOR> { static constexpr int NIN = 3, NOUT = 3; }
 #define GEV_ZS_PTS_TENSOR 4
 #endif
 #ifndef GEV_ZS_COLS_F32
-#define GEV_ZS_COLS_F32 4
+#define GEV_ZS_COLS_F32 8
 #endif
 #ifndef GEV_ZS_PTS_TENSOR_F32
-#define GEV_ZS_PTS_TENSOR_F32 2
+#define GEV_ZS_PTS_TENSOR_F32 8
+#endif
+// the Poisson and vector solves (1 and 3 components) have their own column counts
+#ifndef GEV_ZS_COLS_S
+#define GEV_ZS_COLS_S 4
+#endif
+#ifndef GEV_ZS_COLS_S_F32
+#define GEV_ZS_COLS_S_F32 4
 #endif
 template <typename C> constexpr bool is_f32() { return sizeof(C) == 8; }
-template <int N, typename C> constexpr int zs_cols() {
-    return N >= 768 ? 2 : (is_f32<C>() ? GEV_ZS_COLS_F32 : GEV_ZS_COLS);
+template <int N, int KIND, typename C> constexpr int zs_cols() {
+    constexpr bool T = KIND == ZS_TENSOR || KIND == ZS_TENSOR_SCALAR;
+    if (N >= 768) return 2;
+    if (is_f32<C>()) return T ? GEV_ZS_COLS_F32 : GEV_ZS_COLS_S_F32;
+    return T ? GEV_ZS_COLS : GEV_ZS_COLS_S;
 }
 // The tensor solve prefetches the carried B into registers during the last
 // input transform where the registers allow it; at N = 384 and above 512 the
@@ -82,7 +92,7 @@ template <int N, int KIND, typename C> constexpr int zs_pts() {
     return N >= 768 ? 4 : GEV_ZS_PTS_TENSOR_F32;  // two columns above 512: 1024 threads would spill
 }
 template <int N, int KIND, typename C> constexpr int zs_threads() {
-    int t = N * zs_cols<N, C>() / zs_pts<N, KIND, C>();
+    int t = N * zs_cols<N, KIND, C>() / zs_pts<N, KIND, C>();
     if (t > 1024) t = 1024;
     if (t < 64) t = 64;
     return (t + 63) / 64 * 64;
@@ -108,7 +118,7 @@ using ZArgs = ZArgsT<double2>;
 
 template <int N, int KIND, typename C>
 __global__ __launch_bounds__((zs_threads<N, KIND, C>())) void k_zsolve(ZArgsT<C> A) {
-    constexpr int W = zs_cols<N, C>();
+    constexpr int W = zs_cols<N, KIND, C>();
     constexpr int NT = zs_threads<N, KIND, C>();
     constexpr int NIN = ZSolve<KIND>::NIN, NOUT = ZSolve<KIND>::NOUT;
     constexpr int PTS = (N * W + NT - 1) / NT;
@@ -445,7 +455,7 @@ template <int NC> int launch_x_nc(gev_ctx *c, XArgs A) {
 // ---------------------------------------------------------------- last pass
 
 template <int N, int KIND, typename C> int launch_n(gev_ctx *c, const ZArgsT<C> &A) {
-    constexpr int W = zs_cols<N, C>();
+    constexpr int W = zs_cols<N, KIND, C>();
     const int H = N / 2 + 1;
     hipLaunchKernelGGL((k_zsolve<N, KIND, C>), dim3((H + W - 1) / W, N), dim3(zs_threads<N, KIND, C>()), 0, c->stream,
                        A);
