@@ -146,6 +146,54 @@ __device__ __forceinline__ Rec<double> make_rec(const PQ &v, double rx, double r
     return r;
 }
 
+// The FP32 form's record: the CIC weights from the FP64 position (rounded
+// once), the momentum-dependent factors evaluated in float.
+template <int KIND, bool USE_PHI, bool POW2>
+__device__ __forceinline__ Rec<float> make_rec_f32(const PQ &v, double rx, double ry, double rz, double dx, double Nd,
+                                                   float a, float mass_t) {
+    using K = Kind<KIND>;
+    Rec<float> r;
+    r.up0 = (float)over_dx<POW2>(v.x - rx, dx, Nd);
+    r.up1 = (float)over_dx<POW2>(v.y - ry, dx, Nd);
+    r.up2 = (float)over_dx<POW2>(v.z - rz, dx, Nd);
+    r.e = a;
+    r.f00 = 0.f;
+    r.fij = 0.f;
+    r.w0 = r.w1 = r.w2 = r.w01 = r.w02 = r.w12 = 0.f;
+    const float q0 = (float)v.q0, q1 = (float)v.q1, q2 = (float)v.q2;
+    if (K::QE) {
+        const float fq = q0 * q0 + q1 * q1 + q2 * q2;
+        const float e = sqrtf(fq + a * a), me = mass_t / e;
+        if (K::T00 && KIND != 5 && USE_PHI) {
+            r.e = e;
+            r.f00 = 3.f * e + fq / e;
+        }
+        if (K::TIJ) {
+            r.fij = 4.f + a * a / (fq + a * a);
+            r.w0 = me * q0 * q0;
+            r.w1 = me * q1 * q1;
+            r.w2 = me * q2 * q2;
+            r.w01 = me * q0 * q1;
+            r.w02 = me * q0 * q2;
+            r.w12 = me * q1 * q2;
+        }
+    }
+    if (K::T0I) {
+        r.w0 = mass_t * q0;
+        r.w1 = mass_t * q1;
+        r.w2 = mass_t * q2;
+    }
+    return r;
+}
+
+// the record of the form whose arithmetic type is R
+template <typename R, int KIND, bool USE_PHI, bool POW2>
+__device__ __forceinline__ Rec<R> record(const PQ &v, double rx, double ry, double rz, double dx, double Nd, double a,
+                                         double mass_t) {
+    if constexpr (sizeof(R) == 4) return make_rec_f32<KIND, USE_PHI, POW2>(v, rx, ry, rz, dx, Nd, (float)a, (float)mass_t);
+    else return make_rec<KIND, USE_PHI, POW2>(v, rx, ry, rz, dx, Nd, a, mass_t);
+}
+
 // LDS image of the records (structure of arrays)
 template <int KIND, typename R> struct Stage {
     R up0[CAP], up1[CAP], up2[CAP];
@@ -156,8 +204,8 @@ template <int KIND, typename R> struct Stage {
     R w2[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
     R w01[Kind<KIND>::TIJ ? CAP : 1], w02[Kind<KIND>::TIJ ? CAP : 1], w12[Kind<KIND>::TIJ ? CAP : 1];
 
-    // stores a record evaluated in double, rounded to R
-    __device__ __forceinline__ void put(int i, const Rec<double> &r) {
+    // stores a record (evaluated in double or in R), rounded to R
+    template <typename Q> __device__ __forceinline__ void put(int i, const Rec<Q> &r) {
         using K = Kind<KIND>;
         up0[i] = (R)r.up0;
         up1[i] = (R)r.up1;
@@ -477,7 +525,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                     const double rx = (int)floor(over_dx<POW2>(pq[k].x, dx, Nd)) * dx;
                     const double ry = (int)floor(over_dx<POW2>(pq[k].y, dx, Nd)) * dx;
                     const double rz = (int)floor(over_dx<POW2>(pq[k].z, dx, Nd)) * dx;
-                    st.put((int)r, make_rec<KIND, USE_PHI, POW2>(pq[k], rx, ry, rz, dx, Nd, a, mass_t));
+                    st.put((int)r, record<F, KIND, USE_PHI, POW2>(pq[k], rx, ry, rz, dx, Nd, a, mass_t));
                 }
             }
             __syncthreads();
@@ -610,7 +658,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                     const double rx = wrapi(x0 - 1 + order_slot(px, x0, nsx), N) * dx;
                     const double ry = wrapi(y0 - 1 + order_slot(py, y0, nsy), N) * dx;
                     const double rz = wrapi(z0 - 1 + order_slot(pz, z0, nsz), N) * dx;
-                    st.put((int)r, make_rec<KIND, USE_PHI, POW2>(pq[k], rx, ry, rz, dx, Nd, a, mass_t));
+                    st.put((int)r, record<F, KIND, USE_PHI, POW2>(pq[k], rx, ry, rz, dx, Nd, a, mass_t));
                 }
             }
             __syncthreads();
@@ -669,7 +717,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             for (uint32_t s0 = 0; s0 < n; s0 += (uint32_t)CAP) {
                 const uint32_t m = min((uint32_t)CAP, n - s0);
                 for (uint32_t r = threadIdx.x; r < m; r += TPB)
-                    st.put((int)r, make_rec<KIND, USE_PHI, POW2>(load_pq<KIND>(pos, q, cap, perm[s_gbeg[c] + s0 + r]),
+                    st.put((int)r, record<F, KIND, USE_PHI, POW2>(load_pq<KIND>(pos, q, cap, perm[s_gbeg[c] + s0 + r]),
                                                                   rx, ry, rz, dx, Nd, a, mass_t));
                 __syncthreads();
 #define GEV_REC(X, Y, Z) add_records<KIND, X, Y, Z>(P, st, T, m)

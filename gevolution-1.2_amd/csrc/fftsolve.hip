@@ -368,7 +368,17 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgsT<F, C> A) {
             const int x = 2 * n + h;
             const int64_t i = row * N + x;
             const Nb c = nb_of(x, y, z, N);
-            if constexpr (NC == 6) {
+            if constexpr (NC == 6 && is_f32<C>()) {
+                float T[6], S[6];
+#pragma unroll
+                for (int q = 0; q < 6; q++) T[q] = A.src[q * A.n3v + i];
+                prep_tensor_f32(A.phi, T, c, i, N, (float)A.c0, S);
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    if (h == 0) v[q][k].x = S[q];
+                    else v[q][k].y = S[q];
+                }
+            } else if constexpr (NC == 6) {
                 double T[6], S[6];
 #pragma unroll
                 for (int q = 0; q < 6; q++) T[q] = A.src[q * A.n3v + i];

@@ -96,4 +96,24 @@ __device__ __forceinline__ void prep_tensor(const F *__restrict__ phi, const dou
     S[4] = s;
 }
 
+// prepareFTsource (1) in the FP32 form: the same terms as prep_tensor over the
+// float fields, evaluated in float
+__device__ __forceinline__ void prep_tensor_f32(const float *__restrict__ phi, const float T[6], const Nb &c,
+                                                int64_t i, int N, float coeff, float S[6]) {
+    const float p = phi[i];
+    const float px = phi[L(N, c.xp, c.y, c.z)], mx = phi[L(N, c.xm, c.y, c.z)];
+    const float py = phi[L(N, c.x, c.yp, c.z)], my = phi[L(N, c.x, c.ym, c.z)];
+    const float pz = phi[L(N, c.x, c.y, c.zp)], mz = phi[L(N, c.x, c.y, c.zm)];
+    const float pxy = phi[L(N, c.xp, c.yp, c.z)];
+    const float pxz = phi[L(N, c.xp, c.y, c.zp)];
+    const float pyz = phi[L(N, c.x, c.yp, c.zp)];
+    const float hp = 0.5f * p * p, hx = 0.5f * px * px, hy = 0.5f * py * py, hz = 0.5f * pz * pz;
+    S[0] = coeff * T[0] + 0.5f * (px - mx) * (px - mx);
+    S[3] = coeff * T[3] + 0.5f * (py - my) * (py - my);
+    S[5] = coeff * T[5] + 0.5f * (pz - mz) * (pz - mz);
+    S[1] = coeff * T[1] + (px * py - p * pxy) + hp - hx - hy + 0.5f * pxy * pxy;
+    S[2] = coeff * T[2] + (px * pz - p * pxz) + hp - hx - hz + 0.5f * pxz * pxz;
+    S[4] = coeff * T[4] + (py * pz - p * pyz) + hp - hy - hz + 0.5f * pyz * pyz;
+}
+
 }  // namespace gev_prep

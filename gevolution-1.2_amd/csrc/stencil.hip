@@ -35,12 +35,22 @@ __global__ __launch_bounds__(TPB) void k_prepare_tensor(const F *__restrict__ ph
     if (i >= n3v) return;
     const int64_t r = i / N;
     const Nb c = nb_of((int)(i % N), (int)(r % N), (int)(r / N), N);
-    double T[6], S[6];
+    if constexpr (sizeof(F) == 4) {
+        // FP32 form: the float stencil the fused first FFT pass uses
+        float T[6], S[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) T[k] = Sij[k * n3v + i];
-    prep_tensor(phi, T, c, i, N, coeff, S);
+        for (int k = 0; k < 6; k++) T[k] = Sij[k * n3v + i];
+        prep_tensor_f32(phi, T, c, i, N, (float)coeff, S);
 #pragma unroll
-    for (int k = 0; k < 6; k++) Sij[k * n3v + i] = (F)S[k];
+        for (int k = 0; k < 6; k++) Sij[k * n3v + i] = S[k];
+    } else {
+        double T[6], S[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) T[k] = Sij[k * n3v + i];
+        prep_tensor(phi, T, c, i, N, coeff, S);
+#pragma unroll
+        for (int k = 0; k < 6; k++) Sij[k * n3v + i] = (F)S[k];
+    }
 }
 
 // deterministic two-level sum: fixed per-block tree, then one block over
