@@ -143,12 +143,13 @@ __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *
 // c2c along a strided axis.  Column c = (blockIdx.y, kx0 + w): elements at
 // in[blockIdx.y*os + kx + e*es], e < N.  INV = backward (sign +1).
 template <typename T, int N, bool INV>
-__global__ __launch_bounds__(TPB) void k_c2c_cols(const typename cplx_of<T>::type *in,
+__global__ __launch_bounds__(col_tpb<T>()) void k_c2c_cols(const typename cplx_of<T>::type *in,
                                                   typename cplx_of<T>::type *out, int H,
                                                   int64_t es, int64_t os,
                                                   const typename cplx_of<T>::type *__restrict__ twN,
                                                   int64_t cs) {
     using C = typename cplx_of<T>::type;
+    constexpr int TPB = col_tpb<T>();
     in += blockIdx.z * cs;  // component blockIdx.z
     out += blockIdx.z * cs;
     constexpr int W = col_tile<T, N>();
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(TPB) void k_c2c_cols(const typename cplx_of<T>::typ
         }
     }
     __syncthreads();
-    stages<C, N, W, true, 1>(buf, tw);
+    stages<C, N, W, true, 1, TPB>(buf, tw);
     for (int idx = threadIdx.x; idx < N * W; idx += TPB) {
         const int w = idx % W, e = idx / W;
         if (kx0 + w < H) {
@@ -221,12 +222,12 @@ int run_forward(gev_ctx *ctx, const T *real, typename cplx_of<T>::type *k, int n
         // than one launch over the components)
         for (int c = c0; c < c0 + nc && (passes & 2); c++) {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Y, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(TPB), 0, ctx->stream,
+            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0, ctx->stream,
                                k + c * nkv, k + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
         }
         for (int c = c0; c < c0 + nc && (passes & 4); c++) {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Z, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(TPB), 0, ctx->stream,
+            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0, ctx->stream,
                                k + c * nkv, k + c * nkv, H, (int64_t)P * N, (int64_t)P, tw, (int64_t)0);
         }
     }
@@ -250,7 +251,7 @@ int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int 
             const int nc = ncomp - c0 < MAXC ? ncomp - c0 : MAXC;
             for (int c = c0; c < c0 + nc; c++) {
                 gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
-                hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(TPB), 0, ctx->stream,
+                hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0, ctx->stream,
                                    kk + c * nkv, kk + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
             }
             {
@@ -264,12 +265,12 @@ int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int 
     for (int c = 0; c < ncomp; c++) {
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Z_INV, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(TPB), 0,
+            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0,
                                ctx->stream, k + c * nkv, tmp, H, (int64_t)P * N, (int64_t)P, tw, (int64_t)0);
         }
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(TPB), 0,
+            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0,
                                ctx->stream, tmp, tmp, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
         }
         {

@@ -167,8 +167,19 @@ template <int N> constexpr int x_lines() {
 #ifndef GEV_COL_BYTES
 #define GEV_COL_BYTES 128
 #endif
+#ifndef GEV_COL_BYTES_F32
+#define GEV_COL_BYTES_F32 128
+#endif
+// threads per workgroup of the strided passes (FP64 / FP32 elements)
+#ifndef GEV_COL_TPB
+#define GEV_COL_TPB 256
+#endif
+#ifndef GEV_COL_TPB_F32
+#define GEV_COL_TPB_F32 512
+#endif
+template <typename T> constexpr int col_tpb() { return sizeof(T) == 4 ? GEV_COL_TPB_F32 : GEV_COL_TPB; }
 template <typename T, int N> constexpr int col_tile() {
-    int w = GEV_COL_BYTES / (int)(2 * sizeof(T));
+    int w = (sizeof(T) == 4 ? GEV_COL_BYTES_F32 : GEV_COL_BYTES) / (int)(2 * sizeof(T));
     while (w > 1 && w * N * (int)(2 * sizeof(T)) > 65536) w /= 2;
     return w;
 }
