@@ -459,12 +459,17 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], View
     const float s0 = 1.f - r0, s1 = 1.f - r1, s2 = 1.f - r2;
     double v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
     double e2 = v2 + params[0] * params[0];
-    float g[3];
-    grad_f32(phi, r0, r1, r2, g);
-    const double fac = (v2 + e2) / e2;
-    double gradphi[3] = {g[0] * fac, g[1] * fac, g[2] * fac};
+    // the force in float: (v^2 + e^2) / e^2 and the B term's 1 / (N a^2 e) are
+    // evaluated in FP64 and rounded once
+    float gradphi[3];
+    grad_f32(phi, r0, r1, r2, gradphi);
+    const float fac = (float)((v2 + e2) / e2);
+    gradphi[0] *= fac;
+    gradphi[1] *= fac;
+    gradphi[2] *= fac;
     GEV_LDS_PHASE();
     if (nfield >= 2) {
+        float g[3];
         grad_f32(chi, r0, r1, r2, g);
         gradphi[0] -= g[0];
         gradphi[1] -= g[1];
@@ -498,17 +503,16 @@ __device__ __forceinline__ double update_q(double q[3], const double rd[3], View
                           s0 * r1 * (e0 * B2.raw(0, 1, -1) + e1 * B2.raw(0, 1, 0) + r2 * B2.raw(0, 1, 1)) +
                           r0 * r1 * (e0 * B2.raw(1, 1, -1) + e1 * B2.raw(1, 1, 0) + r2 * B2.raw(1, 1, 1));
         GEV_LDS_PHASE();
-        const double pb0 = a01 * q[1] + a02 * q[2] + a00 * q[0];
-        const double pb1 = a12 * q[2] + a10 * q[0] + a11 * q[1];
-        const double pb2 = a20 * q[0] + a21 * q[1] + a22 * q[2];
-        gradphi[0] += pb0 / params[1] / e2;
-        gradphi[1] += pb1 / params[1] / e2;
-        gradphi[2] += pb2 / params[1] / e2;
+        const float qf0 = (float)q[0], qf1 = (float)q[1], qf2 = (float)q[2];
+        const float inv = (float)(1. / params[1] / e2);
+        gradphi[0] += (a01 * qf1 + a02 * qf2 + a00 * qf0) * inv;
+        gradphi[1] += (a12 * qf2 + a10 * qf0 + a11 * qf1) * inv;
+        gradphi[2] += (a20 * qf0 + a21 * qf1 + a22 * qf2) * inv;
     }
     v2 = 0.;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        q[i] -= over_dx<POW2>(dtau * e2 * gradphi[i], dx, 1.0 / dx);
+        q[i] -= over_dx<POW2>(dtau * e2 * (double)gradphi[i], dx, 1.0 / dx);
         v2 += q[i] * q[i];
     }
     return v2 / params[0] / params[0];
@@ -578,6 +582,11 @@ struct StepArgs {
 // locate, optional kick, optional drift (+ wrap + count into the new cell).
 // QM / PM select the callbacks (0 = GR, 1 = Newton).  The filing
 // (start / perm) must describe the current positions.
+// The FP32 form's particle pass asks for 5 waves per SIMD (96 VGPRs, no
+// spill; 5.2 against 5.5 ms at 512^3)
+#ifndef GEV_KICK_WAVES_F32
+#define GEV_KICK_WAVES_F32 5
+#endif
 // GEV_KICK_WAVES > 0 asks the compiler for that many waves per SIMD
 #ifndef GEV_KICK_WAVES
 #define GEV_KICK_WAVES 0
@@ -589,7 +598,7 @@ struct StepArgs {
 #endif
 
 template <bool KICK, int QM, bool DRIFT, int PM, bool POW2, typename F>
-__global__ __launch_bounds__(TPB) GEV_KICK_OCC void k_update_tiled(
+__global__ __launch_bounds__(TPB, (sizeof(F) == 4 ? GEV_KICK_WAVES_F32 : 1)) GEV_KICK_OCC void k_update_tiled(
     double *__restrict__ pos, double *__restrict__ q, int64_t cap, int N, int ntx, int nty, int ntz,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const F *__restrict__ phi,
     const F *__restrict__ chi, const F *__restrict__ Bi, StepArgs sa,
