@@ -260,8 +260,8 @@ def precision_form_report(gev, box, st, box32, st32, sim, args, ms64):
     pk_err = float(np.max(np.abs(p32[1][low] - p64[1][low]) / np.abs(p64[1][low])))
     print("FP32 form: %.3f ms/cycle (%.2fx), phi error %.2e, P(k) error %.2e"
           % (ms32, ms64 / ms32, phi_err, pk_err), file=sys.stderr, flush=True)
-    return {"form": "fp32 fields, FFT passes and deposit sums; fp64 particles, background, stencil arithmetic "
-                    "and the k-space solves' coefficients and denominators (k->0, zero mode)",
+    return {"form": "fp32 fields, FFT passes, deposit sums and field stencils; fp64 particles, background, "
+                    "momentum factors and the k-space solves' coefficients and denominators (k->0, zero mode)",
             "ms_per_step": round(ms32, 3), "value": round(box.N ** 3 * args.steps / el, 1),
             "speedup_vs_fp64": round(ms64 / ms32, 3),
             "phi_max_rel_err": phi_err, "pk_max_rel_err_below_half_nyquist": pk_err,

@@ -76,9 +76,10 @@ enum { GEV_POS_GR = 0, GEV_POS_NEWTON = 1 }; /* update_pos, update_pos_Newton */
 /* forms of a box (gev_box_desc.form) */
 enum {
     GEV_FORM_FP64 = 0, /* plain FP64: the drop-in                                  */
-    GEV_FORM_FP32 = 1  /* precision form: FP32 field storage, FFT passes and deposit
-                          sums; FP64 particles, background, stencil arithmetic and
-                          the k-space solves' coefficients and denominators */
+    GEV_FORM_FP32 = 1  /* precision form: FP32 field storage, FFT passes, deposit
+                          sums and field stencils; FP64 particles, background,
+                          momentum factors and the k-space solves' coefficients,
+                          denominators and zero mode */
 };
 
 typedef struct {

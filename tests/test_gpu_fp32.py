@@ -2,10 +2,12 @@
 plain FP64 form, which the oracle pins.
 
 The FP32 form stores phi, chi, source, Sij, Bi and the k-space fields as
-float and runs the FFT butterflies and the deposit's per-particle sums in
-float; particles, the background, the stencils' arithmetic and the k-space
-solves (denominators, zero mode) stay FP64.  Its results therefore differ
-from the FP64 form by float rounding, and the bars are stated per quantity:
+float and runs the FFT butterflies, the deposit's sums, the tensor stencil,
+the kick and drift gathers and the tensor solves' field combinations in
+float; the particles, the background, every factor involving a particle's
+momentum, the scalar stencil, and the k-space solves' coefficients,
+denominators and zero mode stay FP64.  Its results therefore differ from the
+FP64 form by float rounding, and the bars are stated per quantity:
   * one FFT round trip and a forward transform: max error <= 2e-6 x max |ref|
     (about 17 ulp of float at the magnitudes involved, log2 N stages);
   * whole cycles from the same state: phi, chi within 1e-4 relative (max
