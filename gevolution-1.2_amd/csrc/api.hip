@@ -112,6 +112,7 @@ static int64_t footprint(int N, int64_t cap, bool fp32) {
     b += fp32 ? N * 8 : 0;          // float twiddles
     b += s.cap * (2 * (6 * 8 + 8) + 2 * 4);  // pos, q, id (two sets), cell, perm
     b += s.n3 * 4 + (s.n3 + 1) * 4 + s.nblk_scan * 4;
+    b += (s.n3 / 256 + 1 + sort_big_cap(s.cap) + 2) * 4;  // the ID sort's lists and counts
     b += s.nred * 8 + 8;
     b += N * (16 + 8 + 16);
     return b;
@@ -120,7 +121,7 @@ static int64_t footprint(int N, int64_t cap, bool fp32) {
 static void free_all(gev_ctx *c) {
     void *ptrs[] = {c->phi, c->chi, c->source, c->Sij, c->Bi, c->scalarFT, c->SijFT, c->BiFT,
                     c->fft_tmp, c->twiddle, c->twiddle_f, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
-                    c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u};
+                    c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u, c->sort_segs, c->sort_big, c->sort_n};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
@@ -215,6 +216,9 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     A((void **)&c->count, s.n3 * 4);
     A((void **)&c->start, (s.n3 + 1) * 4);
     A((void **)&c->scan_part, s.nblk_scan * 4);
+    A((void **)&c->sort_segs, (s.n3 / 256 + 1) * 4);
+    A((void **)&c->sort_big, sort_big_cap(s.cap) * 4);
+    A((void **)&c->sort_n, 8);
     A((void **)&c->red_d, s.nred * 8);
     A((void **)&c->red_u, 64);
     if (ok && hipHostMalloc((void **)&c->host_pinned, 64) != hipSuccess) ok = false;

@@ -72,6 +72,10 @@ struct gev_ctx {
     bool count_clean = false;   // count is all zero (every completed filing leaves it so)
     uint32_t *start = nullptr;  // N^3 + 1 CSR offsets
     uint32_t *scan_part = nullptr;  // scan block partials
+    // the filing's ID sort: a flag byte per segment (256 consecutive cells)
+    // with a cell of more than 4 particles, the listed cells of dense
+    // segments, and their count (sort_n[1])
+    uint32_t *sort_segs = nullptr, *sort_big = nullptr, *sort_n = nullptr;
     int64_t scan_nblk = 0;
 
     // reductions
@@ -118,6 +122,9 @@ template <typename F> inline const F *fld(const double *p) { return reinterpret_
 template <typename C> inline C *cfld(double2 *p) { return reinterpret_cast<C *>(p); }
 template <typename C> inline const C *cfld(const double2 *p) { return reinterpret_cast<const C *>(p); }
 inline int64_t nk(const gev_ctx *c) { return (int64_t)c->P * c->N * c->N; }
+// cells the filing's ID sort lists, at most: cells of more than 4 particles
+// in segments (256 consecutive cells) holding more than 2048 particles
+inline int64_t sort_big_cap(int64_t cap) { return (cap / 2049 + 1) * 256; }
 // component k of a complex field handle (components are nk elements apart in
 // the form's element size)
 inline double2 *kcomp(const gev_ctx *c, double2 *p, int k) {

@@ -152,28 +152,71 @@ __global__ __launch_bounds__(TPB) void k_scan_final(const uint32_t *__restrict__
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry;
 }
 
+// Runs of equal cells among the active lanes of a wavefront (consecutive
+// active lanes with the same c): each lane's offset in its run, and the
+// run's length at its first lane.  Particles come in cell order (storage
+// order is the last filing's), so a dense cell's particles share a few
+// runs, and one atomic per run replaces one per particle.
+struct CellRun {
+    int head;      // lane of the run's first lane
+    uint32_t off;  // this lane's offset in the run
+    uint32_t len;  // run length (valid at the head lane)
+};
+__device__ __forceinline__ CellRun cell_run(uint32_t c) {
+    const uint64_t active = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t below = active & ((1ull << lane) - 1ull);
+    const int prev = below ? 63 - __clzll(below) : -1;  // previous active lane
+    const uint32_t cprev = __shfl(c, prev < 0 ? lane : prev);
+    const bool head = prev < 0 || cprev != c;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+    CellRun r;
+    r.head = 63 - __clzll(upto);
+    r.off = (uint32_t)__popcll(active & ((1ull << lane) - 1ull) & ~((1ull << r.head) - 1ull));
+    // the run ends before the next head (or after the last active lane)
+    const uint64_t later = heads & ~upto;
+    const int end = later ? __ffsll((long long)later) - 1 : 64;
+    r.len = (uint32_t)__popcll(active & ~((1ull << lane) - 1ull) & (end == 64 ? ~0ull : ((1ull << end) - 1ull)));
+    return r;
+}
+
 // every particle takes a place of its cell, counting the cell's count back
 // down (so every completed filing leaves count[] all zero for the next one);
 // the in-cell order is then fixed by the ID sort.  A place outside the
 // cell's CSR range (counts not zero before the counting pass) is dropped.
+// One atomic per run of equal cells in a wavefront.
 __global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ cell, const uint32_t *__restrict__ start,
                                                  uint32_t *__restrict__ count, int64_t n,
                                                  uint32_t *__restrict__ perm) {
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= n) return;
     const uint32_t c = cell[p];
-    const uint32_t b = start[c], k = atomicSub(count + c, 1u);
+    const CellRun r = cell_run(c);
+    uint32_t k = 0;
+    if (r.off == 0) k = atomicSub(count + c, r.len);
+    k = __shfl(k, r.head) - r.off;  // this particle's count value, as if it had counted down alone
+    const uint32_t b = start[c];
     if (k - 1u < start[c + 1] - b) perm[b + k - 1u] = (uint32_t)p;
 }
 
-// ascending ID inside every cell (insertion sort; cells hold few particles)
-__global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__ start,
-                                                    const int64_t *__restrict__ id, int64_t n3v,
-                                                    uint32_t *__restrict__ perm) {
-    const int64_t c = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (c >= n3v) return;
-    const uint32_t b0 = start[c], b1 = start[c + 1];
-    if (b1 - b0 < 2) return;
+// Ascending ID inside every cell.  A workgroup takes TPB consecutive cells,
+// whose particles are one contiguous run of the permutation.  Cells of up to
+// SORT_TINY particles are sorted by their own thread in registers (a stable
+// compare-exchange network).  If larger cells remain and the segment's run
+// fits SORT_SEG entries, their entries are copied to LDS with their IDs,
+// every such particle's rank inside its cell (ID, ties by position) is
+// counted there by one thread per particle, and the entries are written back
+// in rank order.  A denser segment lists its larger cells for k_sort_big (a
+// workgroup per cell).  (A thread-per-cell insertion sort over global memory
+// made dense cells slow: every comparison was two dependent global loads.)
+constexpr uint32_t SORT_TINY = 4;
+constexpr int SORT_SEG = 2048;  // permutation entries of a segment sorted in LDS
+static_assert(TPB <= 256, "a segment's cell index fits 8 bits");
+constexpr int SORT_LDS = 2048;  // particles of a listed cell k_sort_big ranks in LDS
+
+__device__ __forceinline__ void insertion_sort(uint32_t *__restrict__ perm, const int64_t *__restrict__ id,
+                                               uint32_t b0, uint32_t b1) {
     for (uint32_t i = b0 + 1; i < b1; i++) {
         const uint32_t v = perm[i];
         const int64_t key = id[v];
@@ -183,6 +226,153 @@ __global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__
             j--;
         }
         perm[j] = v;
+    }
+}
+
+__device__ __forceinline__ void cswap(uint32_t *v, int64_t *k, int a, int b) {
+    if (k[a] > k[b]) {
+        const int64_t tk = k[a];
+        k[a] = k[b];
+        k[b] = tk;
+        const uint32_t tv = v[a];
+        v[a] = v[b];
+        v[b] = tv;
+    }
+}
+
+// thread per cell: the tiny sorts; a segment with a larger cell is flagged
+// (a flag per segment: one shared list counter would serialise the late,
+// clustered cycles, which flag most segments, on one atomic)
+__global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__ start,
+                                                    const int64_t *__restrict__ id, int64_t n3v,
+                                                    uint32_t *__restrict__ perm, uint8_t *__restrict__ flag) {
+    const int64_t c = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const uint32_t b0 = c < n3v ? start[c] : 0u;
+    const uint32_t m = c < n3v ? start[c + 1] - b0 : 0u;
+    if (m >= 2 && m <= SORT_TINY) {
+        uint32_t v[SORT_TINY];
+        int64_t k[SORT_TINY];
+#pragma unroll
+        for (int i = 0; i < (int)SORT_TINY; i++) v[i] = (uint32_t)i < m ? perm[b0 + i] : 0u;
+#pragma unroll
+        for (int i = 0; i < (int)SORT_TINY; i++) k[i] = (uint32_t)i < m ? id[v[i]] : INT64_MAX;
+        // bubble network: stable (exchanges only on strictly greater keys)
+        cswap(v, k, 0, 1);
+        cswap(v, k, 1, 2);
+        cswap(v, k, 2, 3);
+        cswap(v, k, 0, 1);
+        cswap(v, k, 1, 2);
+        cswap(v, k, 0, 1);
+#pragma unroll
+        for (int i = 0; i < (int)SORT_TINY; i++)
+            if ((uint32_t)i < m) perm[b0 + i] = v[i];
+    }
+    const int big = __syncthreads_or(m > SORT_TINY);
+    if (threadIdx.x == 0) flag[blockIdx.x] = big ? 1 : 0;
+}
+
+// workgroups striding over the segments, taking the flagged ones: the
+// larger cells' entries are ranked in LDS, or, in a segment of more than
+// SORT_SEG entries, listed for k_sort_big
+__global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restrict__ start,
+                                                       const int64_t *__restrict__ id, int64_t n3v,
+                                                       uint32_t *__restrict__ perm, const uint8_t *__restrict__ flag,
+                                                       uint32_t nsegs, uint32_t *__restrict__ big,
+                                                       uint32_t *__restrict__ nbig) {
+    __shared__ int64_t s_key[SORT_SEG];
+    __shared__ uint32_t s_val[SORT_SEG];
+    __shared__ uint32_t s_out[SORT_SEG];
+    __shared__ uint32_t s_cs[TPB + 1];  // cell starts relative to the segment
+    __shared__ uint8_t s_cell[SORT_SEG];  // each entry's cell in the segment
+    for (uint32_t e = blockIdx.x; e < nsegs; e += gridDim.x) {
+        if (!flag[e]) continue;
+        const int64_t c0 = (int64_t)e * TPB;
+        const int ncell = (int)(n3v - c0 < TPB ? n3v - c0 : TPB);
+        const uint32_t s0 = start[c0];
+        for (int t = threadIdx.x; t <= ncell; t += TPB) s_cs[t] = start[c0 + t] - s0;
+        __syncthreads();
+        const uint32_t total = s_cs[ncell];
+        if (total > (uint32_t)SORT_SEG) {
+            // dense segment: its larger cells are ranked one workgroup each
+            if (threadIdx.x < ncell && s_cs[threadIdx.x + 1] - s_cs[threadIdx.x] > SORT_TINY)
+                big[atomicAdd(nbig, 1u)] = (uint32_t)(c0 + threadIdx.x);
+            __syncthreads();  // s_cs is rewritten for the next segment
+            continue;
+        }
+        auto cell_of = [&](uint32_t i) {
+            int lo = 0, hi = ncell - 1;  // the last cell whose start is <= i
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_cs[mid] <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            return lo;
+        };
+        for (uint32_t i = threadIdx.x; i < total; i += TPB) {
+            const int c = cell_of(i);
+            s_cell[i] = (uint8_t)c;
+            if (s_cs[c + 1] - s_cs[c] > SORT_TINY) {
+                const uint32_t v = perm[s0 + i];
+                s_val[i] = v;
+                s_key[i] = id[v];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < total; i += TPB) {
+            const int c = s_cell[i];
+            const uint32_t b = s_cs[c], f = s_cs[c + 1];
+            if (f - b <= SORT_TINY) continue;
+            const int64_t k = s_key[i];
+            uint32_t r = 0;
+            for (uint32_t j = b; j < f; j++) {
+                const int64_t kj = s_key[j];
+                r += (kj < k || (kj == k && j < i)) ? 1u : 0u;
+            }
+            s_out[b + r] = s_val[i];
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < total; i += TPB) {
+            const int c = s_cell[i];
+            if (s_cs[c + 1] - s_cs[c] > SORT_TINY) perm[s0 + i] = s_out[i];
+        }
+        __syncthreads();  // the next segment rewrites s_cs and the copies
+    }
+}
+
+// One workgroup per listed cell (grid-stride over the list): every particle's
+// rank among the cell's IDs (ties by position) from an LDS copy, then one
+// store each.  Cells above SORT_LDS particles fall back to one thread.
+__global__ __launch_bounds__(TPB) void k_sort_big(const uint32_t *__restrict__ start, const int64_t *__restrict__ id,
+                                                  uint32_t *__restrict__ perm, const uint32_t *__restrict__ big,
+                                                  const uint32_t *__restrict__ nbig) {
+    // (nbig: the big list's count)
+    __shared__ int64_t s_key[SORT_LDS];
+    __shared__ uint32_t s_val[SORT_LDS];
+    const uint32_t nb = *nbig;
+    for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
+        const uint32_t c = big[e];
+        const uint32_t b0 = start[c], b1 = start[c + 1], m = b1 - b0;
+        if (m > (uint32_t)SORT_LDS) {
+            if (threadIdx.x == 0) insertion_sort(perm, id, b0, b1);
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t i = threadIdx.x; i < m; i += TPB) {
+            const uint32_t v = perm[b0 + i];
+            s_val[i] = v;
+            s_key[i] = id[v];
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < m; i += TPB) {
+            const int64_t k = s_key[i];
+            uint32_t r = 0;
+            for (uint32_t j = 0; j < m; j++) {
+                const int64_t kj = s_key[j];
+                r += (kj < k || (kj == k && j < i)) ? 1u : 0u;
+            }
+            perm[b0 + r] = s_val[i];
+        }
+        __syncthreads();  // the next cell overwrites the LDS copy
     }
 }
 
@@ -867,8 +1057,25 @@ static int finish_filing(gev_ctx *ctx) {
     }
     {
         ScopedTimer t(ctx, "filing_sort");
+        // sort_n[1]: the number of listed cells
+        int rc = check_hip(ctx, hipMemsetAsync(ctx->sort_n, 0, 2 * sizeof(uint32_t), ctx->stream), "sort list clear");
+        if (rc) return rc;
         hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(n3v, TPB)), dim3(TPB), 0, ctx->stream,
-                           ctx->start, ctx->id, n3v, ctx->perm);
+                           ctx->start, ctx->id, n3v, ctx->perm, reinterpret_cast<uint8_t *>(ctx->sort_segs));
+    }
+    {
+        ScopedTimer t(ctx, "filing_sort_segments");
+        // a quarter of all segments' workgroups, grid-striding over the listed ones: the
+        // late, clustered cycles list most segments and need the parallelism
+        const unsigned nsw = (unsigned)(grid_for(n3v, TPB) / 4 + 1024);
+        hipLaunchKernelGGL(k_sort_segments, dim3(nsw), dim3(TPB), 0, ctx->stream, ctx->start, ctx->id, n3v,
+                           ctx->perm, reinterpret_cast<const uint8_t *>(ctx->sort_segs),
+                           (uint32_t)grid_for(n3v, TPB), ctx->sort_big, ctx->sort_n + 1);
+    }
+    {
+        ScopedTimer t(ctx, "filing_sort_dense_cells");
+        hipLaunchKernelGGL(k_sort_big, dim3(1024), dim3(TPB), 0, ctx->stream, ctx->start, ctx->id, ctx->perm,
+                           ctx->sort_big, ctx->sort_n + 1);
     }
     const int rc = check_hip(ctx, hipGetLastError(), "filing launch");
     if (!rc) ctx->count_clean = true;  // the scatter counted every cell back to zero

@@ -212,6 +212,42 @@ def test_deposit_edge_cases(gev, orc):
         assert np.array_equal(b.download_field(gev.SIJ), orc.projection_Tij_project(*args, a=0.5, phi=phi))
 
 
+def test_dense_cells_filing_and_deposit(gev, orc):
+    """Cells of every size class of the filing's ID sort (tiny, one
+    wavefront, one workgroup in LDS, and above the LDS size) with shuffled
+    IDs: the filing equals the oracle's (ascending ID inside each cell), and
+    the deposit, whose sums run in that order, is bit-exact -- before and
+    after a drift re-files the particles."""
+    N = 16
+    rng = np.random.default_rng(5)
+    groups = [(3, 0.41), (20, 0.27), (100, 0.66), (2500, 0.09), (1, 0.8)]
+    pos = [np.tile([[c0, c0 + 0.01, c0 + 0.02]], (n, 1)) + rng.uniform(0, 0.05 / N, size=(n, 3))
+           for n, c0 in groups]
+    pos = np.concatenate(pos + [rng.uniform(0, 1, size=(300, 3))])
+    n = len(pos)
+    ids = rng.permutation(np.arange(10 * n, dtype=np.int64))[:n]
+    q = rng.standard_normal((n, 3)) * 1e-3
+    phi = smooth_field(N, rng, 1e-5)
+    chi = smooth_field(N, rng, 1e-6)
+    Bi = smooth_field(N, rng, 1e-8, ncomp=3)
+    a = 0.3
+    with gev.Box(N, n) as b:
+        b.upload_particles(pos, q, ids, 1e-3)
+        b.upload_field(gev.PHI, phi)
+        b.upload_field(gev.CHI, chi)
+        b.upload_field(gev.BI, Bi)
+        for step in range(2):
+            dpos, dq, dids = b.download_particles()
+            check_filing(gev, orc, b, N, dpos, dids)
+            b.projection_init(gev.SOURCE)
+            b.projection_init(gev.SIJ)
+            b.projection_T00_Tij_project(a, 1.0)
+            args = (N, dpos, dq, dids, 1e-3)
+            assert np.array_equal(b.download_field(gev.SOURCE), orc.projection_T00_project(*args, a=a, phi=phi))
+            assert np.array_equal(b.download_field(gev.SIJ), orc.projection_Tij_project(*args, a=a, phi=phi))
+            b.moveParticles(gev.POS_GR, 0.002, 3, [a, a * a * N])
+
+
 def test_empty_box(gev):
     N = 16
     with gev.Box(N, 0) as b:
