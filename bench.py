@@ -16,11 +16,16 @@ is timed once and reported beside the cycle throughput ("ic_generator").
 `--state lattice` instead starts from the bare cell-centred lattice with a
 +-0.05-cell jitter and small random momenta (the CPU baseline's sample).
 
-Output: one JSON line.  `value` is particle-steps/s with everything resident
-in HBM; `roofline` is the kernel with the largest share of the timed region,
-its algorithmic bytes per launch divided by its mean duration (HIP events on
-the context stream); `cpu_baseline` is the CPU oracle (the restated
-reference algorithm, OpenMP) on a bounded sample of the same workload.
+Output: one JSON line.  `value` is particle-steps/s over the timed cycles
+(z = 100 down to about z = 60) with everything resident in HBM; `roofline`
+is the kernel with the largest share of the timed region, its algorithmic
+bytes per launch divided by its mean duration (HIP events on the context
+stream); `cpu_baseline` is the CPU oracle (the restated reference
+algorithm, OpenMP) on a bounded sample of the same workload;
+`precision_form` times the FP32 form over the same cycles from the same
+state; `full_run` runs the whole configuration, z = 100 to z = 0 (117
+cycles), in both forms: the time to z = 0 and the FP32 form's errors at
+z = 10, 1 and 0.
 """
 import argparse
 import json
@@ -271,6 +276,59 @@ def precision_form_report(gev, box, st, box32, st32, sim, args, ms64):
             "kernels": kernel_breakdown(k32, args.steps, N, N ** 3, 4) if k32 else {}}
 
 
+def full_run_report(gev, box, box32, N, boxsize, seed):
+    """The whole run of the workload: generated ICs at z = 100 evolved to
+    z = 0 in both forms (one cycle at a time, synchronised, measurements
+    outside the timed cycles).  Late cycles cost more than early ones (the
+    clustered particles make the deposit and the filing slower), so this is
+    the time a user of the configuration waits.  The FP32 form's phi and
+    matter power spectrum errors against the FP64 form are stated at z = 10,
+    1 and 0."""
+    sim, st, _, _, _ = ic_state(gev, box, N, boxsize, seed)
+    st32 = None
+    if box32 is not None:
+        box32.copy_state_from(box)
+        st32 = gev.CycleState(st.a, st.tau, st.dtau, st.dtau_old, st.maxvel, st.T00hom, st.phi_k0, st.cycle, 0)
+    t64 = t32 = 0.0
+    points = []
+    for zt in (10.0, 1.0, 0.0):
+        while 1.0 / st.a - 1.0 > zt + 1e-12:
+            box.synchronize()
+            t0 = time.perf_counter()
+            box.run_cycles(sim, st, 1)
+            box.synchronize()
+            t64 += time.perf_counter() - t0
+            if box32 is not None:
+                t0 = time.perf_counter()
+                box32.run_cycles(sim, st32, 1)
+                box32.synchronize()
+                t32 += time.perf_counter() - t0
+        pt = {"z": round(1.0 / st.a - 1.0, 4), "cycle": st.cycle, "fp64_s": round(t64, 3)}
+        if box32 is not None:
+            assert st32.a == st.a
+            pt["fp32_s"] = round(t32, 3)
+            phi64 = box.download_field(gev.PHI)
+            phi32 = box32.download_field(gev.PHI)
+            pt["phi_max_rel_err"] = float(np.max(np.abs(phi32 - phi64)) / np.max(np.abs(phi64)))
+            del phi64, phi32
+            p64 = delta_spectrum(gev, box, st.a, N // 2)
+            p32 = delta_spectrum(gev, box32, st32.a, N // 2)
+            low = (p64[4] > 0) & (p64[0] < 0.5 * np.pi * N)
+            pt["pk_max_rel_err_below_half_nyquist"] = float(np.max(np.abs(p32[1][low] - p64[1][low]) /
+                                                                   np.abs(p64[1][low])))
+        points.append(pt)
+        print("full run: z = %.3f after %d cycles, %.2f s FP64%s" % (pt["z"], pt["cycle"], t64,
+              (", %.2f s FP32" % t32) if box32 is not None else ""), file=sys.stderr, flush=True)
+    cyc = st.cycle
+    out = {"from_z": 100.0, "cycles": cyc, "time_to_z0_s": round(t64, 3),
+           "ms_per_step": round(1e3 * t64 / cyc, 3), "value": round(N ** 3 * cyc / t64, 1),
+           "unit": "particle-steps/s", "checkpoints": points}
+    if box32 is not None:
+        out["fp32"] = {"time_to_z0_s": round(t32, 3), "ms_per_step": round(1e3 * t32 / cyc, 3),
+                       "value": round(N ** 3 * cyc / t32, 1), "speedup_vs_fp64": round(t64 / t32, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,6 +348,8 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-precision-form", action="store_true",
                     help="skip the FP32 form's timing and error report")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the whole run from z = 100 to z = 0 (both forms)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 1 GiB device copies (scale for the PMC byte counters)")
     args = ap.parse_args()
@@ -403,6 +463,8 @@ def main():
         "precision_form": (precision_form_report(gev, box, st, box32, st32, sim, args, ms_per_step)
                            if box32 is not None else None),
     }
+    if not args.no_full_run and args.state == "ic":
+        out["full_run"] = full_run_report(gev, box, box32, N, boxsize, args.seed)
     if box32 is not None:
         box32.close()
     box.close()

@@ -44,7 +44,12 @@ constexpr int TX = GEV_DEP_TX, TY = GEV_DEP_TY, TZ = GEV_DEP_TZ;
 constexpr int TPB = TX * TY * TZ;   // one target cell per thread
 constexpr int SX = TX + 1, SY = TY + 1, SZ = TZ + 1;
 constexpr int NSRC = SX * SY * SZ;  // source cells of a tile (425 for 16x4x4)
-constexpr int CAP = GEV_DEP_CAP;    // particle records per chunk
+// particle records per chunk: the FP32 form's records are half the size, so
+// it stages more of them in the same LDS (clustered tiles take fewer chunks)
+#ifndef GEV_DEP_CAP_F32
+#define GEV_DEP_CAP_F32 448
+#endif
+template <typename R> constexpr int dep_cap() { return sizeof(R) == 4 ? GEV_DEP_CAP_F32 : GEV_DEP_CAP; }
 static_assert(NSRC <= 2 * TPB, "two source cells per thread");
 static_assert(TPB % 64 == 0 && TPB <= 1024, "whole wavefronts");
 
@@ -196,13 +201,13 @@ __device__ __forceinline__ Rec<R> record(const PQ &v, double rx, double ry, doub
 
 // LDS image of the records (structure of arrays)
 template <int KIND, typename R> struct Stage {
-    R up0[CAP], up1[CAP], up2[CAP];
-    R e[Kind<KIND>::T00 ? CAP : 1], f00[Kind<KIND>::T00 ? CAP : 1];
-    R fij[Kind<KIND>::TIJ ? CAP : 1];
-    R w0[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
-    R w1[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
-    R w2[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? CAP : 1];
-    R w01[Kind<KIND>::TIJ ? CAP : 1], w02[Kind<KIND>::TIJ ? CAP : 1], w12[Kind<KIND>::TIJ ? CAP : 1];
+    R up0[dep_cap<R>()], up1[dep_cap<R>()], up2[dep_cap<R>()];
+    R e[Kind<KIND>::T00 ? dep_cap<R>() : 1], f00[Kind<KIND>::T00 ? dep_cap<R>() : 1];
+    R fij[Kind<KIND>::TIJ ? dep_cap<R>() : 1];
+    R w0[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? dep_cap<R>() : 1];
+    R w1[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? dep_cap<R>() : 1];
+    R w2[(Kind<KIND>::TIJ || Kind<KIND>::T0I) ? dep_cap<R>() : 1];
+    R w01[Kind<KIND>::TIJ ? dep_cap<R>() : 1], w02[Kind<KIND>::TIJ ? dep_cap<R>() : 1], w12[Kind<KIND>::TIJ ? dep_cap<R>() : 1];
 
     // stores a record (evaluated in double or in R), rounded to R
     template <typename Q> __device__ __forceinline__ void put(int i, const Rec<Q> &r) {
@@ -385,6 +390,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const F *__restrict__ phi, F *__restrict__ out_s, F *__restrict__ out_t, int N,
     int ntx, int nty, int ntz, double a, double mass_s, double mass_t) {
     using K = Kind<KIND>;
+    constexpr int CAP = dep_cap<F>();
     __shared__ Stage<KIND, F> st;
     __shared__ uint32_t s_off[NSRC + 1];
     __shared__ uint32_t s_gbeg[NSRC];
