@@ -50,6 +50,11 @@ constexpr int NSRC = SX * SY * SZ;  // source cells of a tile (425 for 16x4x4)
 #define GEV_DEP_CAP_F32 448
 #endif
 template <typename R> constexpr int dep_cap() { return sizeof(R) == 4 ? GEV_DEP_CAP_F32 : GEV_DEP_CAP; }
+// the FP32 form's tiles above this many particles in their source region
+// take the fixed-point particle-parallel path
+#ifndef GEV_DEP_FIXED_MIN
+#define GEV_DEP_FIXED_MIN 448
+#endif
 static_assert(NSRC <= 2 * TPB, "two source cells per thread");
 static_assert(TPB % 64 == 0 && TPB <= 1024, "whole wavefronts");
 
@@ -391,7 +396,17 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     int ntx, int nty, int ntz, double a, double mass_s, double mass_t) {
     using K = Kind<KIND>;
     constexpr int CAP = dep_cap<F>();
-    __shared__ Stage<KIND, F> st;
+    // the records, or (FP32 form, dense tiles) the fixed-point accumulators
+    // and the target values
+    union StageOrFixed {
+        Stage<KIND, F> st;
+        struct {
+            unsigned long long acc[7 * TPB];
+            float T[4 * TPB];
+        } fx;
+    };
+    __shared__ StageOrFixed su;
+    Stage<KIND, F> &st = su.st;
     __shared__ uint32_t s_off[NSRC + 1];
     __shared__ uint32_t s_gbeg[NSRC];
     __shared__ uint32_t s_cnt[NSRC];
@@ -606,6 +621,163 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
         if (c + 1 < nsrc) s_off[c + 1] = excl + c0;
         if (threadIdx.x == TPB - 1) s_off[nsrc] = before + incl;
         __syncthreads();
+    }
+
+    // FP32 form, a tile whose source region holds more than CAP particles
+    // (the clustered tiles of late cycles): particle-parallel accumulation
+    // into 64-bit fixed point in LDS.  In the gather each target thread walks
+    // its 8 source cells' particles, so a dense cell keeps its wavefront busy
+    // while the other lanes idle; here every thread takes particles in turn
+    // and adds each one's contributions to its (up to 8) targets with integer
+    // atomics, whose sums do not depend on the order: the result is
+    // bit-reproducible.  Every contribution is a float rounded once to the
+    // tile's fixed-point scale for its component, a power of two chosen so
+    // that the sum of all the tile's contribution bounds stays below 2^62.
+    if constexpr (sizeof(F) == 4 && (KIND == 1 || KIND == 2 || KIND == 3)) {
+        if (s_off[nsrc] > (uint32_t)max(CAP, GEV_DEP_FIXED_MIN)) {
+            constexpr int NCOMP = 7;  // T00, T_00, T_01, T_02, T_11, T_12, T_22
+            unsigned long long *s_acc = su.fx.acc;
+            float *s_T = su.fx.T;  // ph, plq01, plq02, plq12 of every target
+            __shared__ float s_max[TPB / 64][NCOMP];
+            __shared__ int s_k[NCOMP];
+            const uint32_t total = s_off[nsrc];
+            for (int i = threadIdx.x; i < NCOMP * TPB; i += TPB) s_acc[i] = 0ull;
+            s_T[threadIdx.x] = T.ph;
+            s_T[TPB + threadIdx.x] = T.plq01;
+            s_T[2 * TPB + threadIdx.x] = T.plq02;
+            s_T[3 * TPB + threadIdx.x] = T.plq12;
+            // the record r of the region: source cell, slot, particle
+            auto locate = [&](uint32_t r, int &c, int &sx, int &sy, int &sz, uint32_t &pidx) {
+                int lo = 0, hi = nsrc - 1;  // the last source cell whose offset is <= r
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= r) lo = mid;
+                    else hi = mid - 1;
+                }
+                c = lo;
+                const int px = c % nsx, py = (c / nsx) % nsy, pz = c / (nsx * nsy);
+                sx = order_slot(px, x0, nsx);
+                sy = order_slot(py, y0, nsy);
+                sz = order_slot(pz, z0, nsz);
+                pidx = perm[s_gbeg[c] + (r - s_off[c])];
+            };
+            // pass 1: per-component bounds of one contribution, maximised over the tile
+            float phm = active ? fabsf(T.ph) : 0.f;
+            float plm = active ? fmaxf(fabsf(T.plq01), fmaxf(fabsf(T.plq02), fabsf(T.plq12))) : 0.f;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                phm = fmaxf(phm, __shfl_xor(phm, off));
+                plm = fmaxf(plm, __shfl_xor(plm, off));
+            }
+            if ((threadIdx.x & 63) == 0) {
+                s_max[threadIdx.x >> 6][0] = phm;
+                s_max[threadIdx.x >> 6][1] = plm;
+            }
+            __syncthreads();
+            phm = s_max[0][0];
+            plm = s_max[0][1];
+#pragma unroll
+            for (int w = 1; w < TPB / 64; w++) {
+                phm = fmaxf(phm, s_max[w][0]);
+                plm = fmaxf(plm, s_max[w][1]);
+            }
+            __syncthreads();
+            float bmax[NCOMP] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            const float ms = (float)mass_s;
+            for (uint32_t r = threadIdx.x; r < total; r += TPB) {
+                int c, sx, sy, sz;
+                uint32_t pidx;
+                locate(r, c, sx, sy, sz, pidx);
+                const PQ v = load_pq<KIND>(pos, q, cap, pidx);
+                const double rx = wrapi(x0 - 1 + sx, N) * dx, ry = wrapi(y0 - 1 + sy, N) * dx;
+                const double rz = wrapi(z0 - 1 + sz, N) * dx;
+                const Rec<float> R = make_rec_f32<KIND, USE_PHI, POW2>(v, rx, ry, rz, dx, Nd, (float)a, (float)mass_t);
+                if (K::T00) bmax[0] = fmaxf(bmax[0], fabsf(ms * (R.e + R.f00 * phm)));
+                if (K::TIJ) {
+                    const float dg = 1.f + R.fij * phm, od = 1.f + R.fij * 0.25f * plm;
+                    bmax[1] = fmaxf(bmax[1], fabsf(R.w0 * dg));
+                    bmax[2] = fmaxf(bmax[2], fabsf(R.w01 * od));
+                    bmax[3] = fmaxf(bmax[3], fabsf(R.w02 * od));
+                    bmax[4] = fmaxf(bmax[4], fabsf(R.w1 * dg));
+                    bmax[5] = fmaxf(bmax[5], fabsf(R.w12 * od));
+                    bmax[6] = fmaxf(bmax[6], fabsf(R.w2 * dg));
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < NCOMP; cc++) {
+                float m = bmax[cc];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+                if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6][cc] = m;
+            }
+            __syncthreads();
+            if (threadIdx.x < NCOMP) {
+                float m = s_max[0][threadIdx.x];
+#pragma unroll
+                for (int w = 1; w < TPB / 64; w++) m = fmaxf(m, s_max[w][threadIdx.x]);
+                // scale 2^k with total * bound * 2^k < 2^61
+                int e = 0;
+                (void)frexp((double)m * (double)total, &e);  // m * total < 2^e
+                // (clamped to float's exponent range: a component whose bound is
+                // below 2^-59 / total is scaled by 2^120 and keeps float precision)
+                s_k[threadIdx.x] = m > 0.f ? max(-100, min(120, 61 - e)) : 0;
+            }
+            __syncthreads();
+            float sc[NCOMP];
+#pragma unroll
+            for (int cc = 0; cc < NCOMP; cc++) sc[cc] = ldexpf(1.f, s_k[cc]);
+            // pass 2: every record's contributions to its in-tile targets
+            for (uint32_t r = threadIdx.x; r < total; r += TPB) {
+                int c, sx, sy, sz;
+                uint32_t pidx;
+                locate(r, c, sx, sy, sz, pidx);
+                const PQ v = load_pq<KIND>(pos, q, cap, pidx);
+                const double rx = wrapi(x0 - 1 + sx, N) * dx, ry = wrapi(y0 - 1 + sy, N) * dx;
+                const double rz = wrapi(z0 - 1 + sz, N) * dx;
+                const Rec<float> R = make_rec_f32<KIND, USE_PHI, POW2>(v, rx, ry, rz, dx, Nd, (float)a, (float)mass_t);
+#pragma unroll
+                for (int d = 0; d < 8; d++) {
+                    const int ex = d & 1, ey = (d >> 1) & 1, ez = d >> 2;
+                    const int tx_ = sx - 1 + ex, ty_ = sy - 1 + ey, tz_ = sz - 1 + ez;
+                    if (tx_ < 0 || tx_ >= nx || ty_ < 0 || ty_ >= ny || tz_ < 0 || tz_ >= nz) continue;
+                    const int tl = tx_ + TX * (ty_ + TY * tz_);
+                    const float W0 = ex ? R.up0 : 1.f - R.up0;
+                    const float W1 = ey ? R.up1 : 1.f - R.up1;
+                    const float W2 = ez ? R.up2 : 1.f - R.up2;
+                    const float ph = s_T[tl];
+                    auto add = [&](int cc, float x) __attribute__((always_inline)) {
+                        atomicAdd(s_acc + cc * TPB + tl, (unsigned long long)__float2ll_rn(x * sc[cc]));
+                    };
+                    if (K::T00) add(0, W0 * W1 * W2 * (R.e + R.f00 * ph) * ms);
+                    if (K::TIJ) {
+                        const float dg = 1.f + R.fij * ph;
+                        add(1, R.w0 * W0 * W1 * W2 * dg);
+                        add(4, R.w1 * W0 * W1 * W2 * dg);
+                        add(6, R.w2 * W0 * W1 * W2 * dg);
+                        if (ex == 0 && ey == 0) add(2, R.w01 * W2 * (1.f + R.fij * 0.25f * s_T[TPB + tl]));
+                        if (ex == 0 && ez == 0) add(3, R.w02 * W1 * (1.f + R.fij * 0.25f * s_T[2 * TPB + tl]));
+                        if (ey == 0 && ez == 0) add(5, R.w12 * W0 * (1.f + R.fij * 0.25f * s_T[3 * TPB + tl]));
+                    }
+                }
+            }
+            __syncthreads();
+            if (active) {
+                auto val = [&](int cc) {
+                    return (float)ldexp((double)(long long)s_acc[cc * TPB + threadIdx.x], -s_k[cc]);
+                };
+                if (K::T00) A.s00 += val(0);
+                if (K::TIJ) {
+                    A.d0 += val(1);
+                    A.o01 += val(2);
+                    A.o02 += val(3);
+                    A.d1 += val(4);
+                    A.o12 += val(5);
+                    A.d2 += val(6);
+                }
+                store_target<KIND>(A, out_s, out_t, t, n3v);
+            }
+            return;
+        }
     }
 
     // 3. chunks of whole source cells, in site order (one chunk in the

@@ -245,6 +245,49 @@ def test_copy_state_and_lcdm_power_spectrum(gev):
     b32.close()
 
 
+def test_fp32_dense_tiles(gev):
+    """Tiles whose source region holds more particles than the LDS records
+    (a cell of 2500 particles, cells of 100 and 20) take the fixed-point
+    particle-parallel deposit in the FP32 form: within float rounding of the
+    FP64 deposit, bit-reproducible, and the same for the fused T00+Tij call
+    as for the two separate calls."""
+    N = 16
+    rng = np.random.default_rng(5)
+    groups = [(3, 0.41), (20, 0.27), (100, 0.66), (2500, 0.09), (1, 0.8)]
+    pos = [np.tile([[c0, c0 + 0.01, c0 + 0.02]], (n, 1)) + rng.uniform(0, 0.05 / N, size=(n, 3))
+           for n, c0 in groups]
+    pos = np.concatenate(pos + [rng.uniform(0, 1, size=(300, 3))])
+    n = len(pos)
+    ids = rng.permutation(np.arange(10 * n, dtype=np.int64))[:n]
+    q = rng.standard_normal((n, 3)) * 1e-3
+    phi = rng.standard_normal((N, N, N)) * 1e-5
+    a = 0.3
+    res = {}
+    for form in (gev.FORM_FP64, gev.FORM_FP32, gev.FORM_FP32):
+        with gev.Box(N, n, form=form) as b:
+            b.upload_particles(pos, q, ids, 1e-3)
+            b.upload_field(gev.PHI, phi)
+            b.projection_init(gev.SOURCE)
+            b.projection_init(gev.SIJ)
+            b.projection_T00_Tij_project(a, 1.0)
+            fused = (b.download_field(gev.SOURCE), b.download_field(gev.SIJ))
+            b.projection_init(gev.SOURCE)
+            b.projection_T00_project(a, True, 1.0)
+            b.projection_init(gev.SIJ)
+            b.projection_Tij_project(a, True, 1.0)
+            sep = (b.download_field(gev.SOURCE), b.download_field(gev.SIJ))
+        res.setdefault(form, []).append((fused, sep))
+    (f64, _), = res[gev.FORM_FP64]
+    (f32a, s32a), (f32b, _) = res[gev.FORM_FP32]
+    for x, y in zip(f32a, f32b):
+        assert np.array_equal(x, y)  # run to run
+    for x, y in zip(f32a, s32a):
+        assert np.array_equal(x, y)  # fused = separate
+    assert rel(f32a[0], f64[0]) <= 1e-6, rel(f32a[0], f64[0])
+    for c in range(6):
+        assert rel(f32a[1][c], f64[1][c]) <= 1e-5, (c, rel(f32a[1][c], f64[1][c]))
+
+
 def test_fp32_form_arguments(gev):
     with pytest.raises(gev.GevError):
         gev.Box(32, 8, form=7)
