@@ -271,9 +271,14 @@ __global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__
     if (threadIdx.x == 0) flag[blockIdx.x] = big ? 1 : 0;
 }
 
-// workgroups striding over the segments, taking the flagged ones: the
-// larger cells' entries are ranked in LDS, or, in a segment of more than
-// SORT_SEG entries, listed for k_sort_big
+// Workgroup b takes the flagged ones among segments b + k * gridDim.x,
+// k < SORT_SEG_BATCH (its first wave reads their flags at once): the larger
+// cells' entries are ranked in LDS, or, in a segment of more than SORT_SEG
+// entries, listed for k_sort_big.  The stride spreads neighbouring (and so
+// similarly clustered) segments over the workgroups.
+constexpr int SORT_SEG_BATCH = 32;
+static_assert(SORT_SEG_BATCH <= 64, "one wavefront reads a batch's flags");
+
 __global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restrict__ start,
                                                        const int64_t *__restrict__ id, int64_t n3v,
                                                        uint32_t *__restrict__ perm, const uint8_t *__restrict__ flag,
@@ -284,8 +289,16 @@ __global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restric
     __shared__ uint32_t s_out[SORT_SEG];
     __shared__ uint32_t s_cs[TPB + 1];  // cell starts relative to the segment
     __shared__ uint8_t s_cell[SORT_SEG];  // each entry's cell in the segment
-    for (uint32_t e = blockIdx.x; e < nsegs; e += gridDim.x) {
-        if (!flag[e]) continue;
+    __shared__ unsigned long long s_mask;
+    if (threadIdx.x < 64) {
+        const uint32_t e = blockIdx.x + threadIdx.x * gridDim.x;
+        const bool f = threadIdx.x < SORT_SEG_BATCH && e < nsegs && flag[e] != 0;
+        const unsigned long long m = __ballot(f);
+        if (threadIdx.x == 0) s_mask = m;
+    }
+    __syncthreads();
+    for (unsigned long long mask = s_mask; mask != 0ull; mask &= mask - 1ull) {
+        const uint32_t e = blockIdx.x + (uint32_t)(__ffsll((long long)mask) - 1) * gridDim.x;
         const int64_t c0 = (int64_t)e * TPB;
         const int ncell = (int)(n3v - c0 < TPB ? n3v - c0 : TPB);
         const uint32_t s0 = start[c0];
@@ -1065,9 +1078,9 @@ static int finish_filing(gev_ctx *ctx) {
     }
     {
         ScopedTimer t(ctx, "filing_sort_segments");
-        // a quarter of all segments' workgroups, grid-striding over the listed ones: the
-        // late, clustered cycles list most segments and need the parallelism
-        const unsigned nsw = (unsigned)(grid_for(n3v, TPB) / 4 + 1024);
+        // SORT_SEG_BATCH segments per workgroup: the late, clustered cycles flag
+        // most segments and need the parallelism; early ones flag almost none
+        const unsigned nsw = (unsigned)((grid_for(n3v, TPB) + SORT_SEG_BATCH - 1) / SORT_SEG_BATCH);
         hipLaunchKernelGGL(k_sort_segments, dim3(nsw), dim3(TPB), 0, ctx->stream, ctx->start, ctx->id, n3v,
                            ctx->perm, reinterpret_cast<const uint8_t *>(ctx->sort_segs),
                            (uint32_t)grid_for(n3v, TPB), ctx->sort_big, ctx->sort_n + 1);
