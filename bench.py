@@ -25,7 +25,10 @@ algorithm, OpenMP) on a bounded sample of the same workload;
 `precision_form` times the FP32 form over the same cycles from the same
 state; `full_run` runs the whole configuration, z = 100 to z = 0 (117
 cycles), in both forms: the time to z = 0 and the FP32 form's errors at
-z = 10, 1 and 0.
+z = 10, 1 and 0; `serving` serves a Poisson trace of whole simulations
+(64^3 to 256^3, GR and Newtonian, z = 100 to 0) with overlapping requests
+and reused boxes, against back-to-back plain runs of the same trace: p50 /
+p99 time to completion and aggregate particle-steps/s of both.
 """
 import argparse
 import json
@@ -329,6 +332,51 @@ def full_run_report(gev, box, box32, N, boxsize, seed):
     return out
 
 
+def serving_report(args):
+    """Whole simulations served on one GPU (serving.py) against back-to-back
+    plain FP64 runs on the same Poisson arrival trace: request grids from
+    {64, 96, 128, 192, 256}^3 (weights .3/.25/.2/.15/.1), half GR and half
+    Newtonian, Omega_m and A_s drawn per request, each generated on the
+    device and run from z = 100 to z = 0.  The plain runs take one request at
+    a time on a box of their own; the served runs overlap up to
+    `serving_slots` requests and reuse idle boxes.  Every request's delta
+    power spectrum and phi sum must be identical in both."""
+    import gevpm as gev
+    import serving as sv
+    tpl = gev.load_homogeneous_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
+    be = sv.GevBackend(template=tpl)
+    trace = sv.poisson_trace(args.serving_requests, args.serving_rate, 2024)
+    # first use of each grid's kernels and allocations, outside both timings
+    with sv.Server(be, sv.default_budget(), slots=1, reuse=False) as s:
+        s.serve([sv.Request(-1 - i, g) for i, g in enumerate(sorted({r.ngrid for r in trace}))])
+    with sv.Server(be, sv.default_budget(), slots=1, reuse=False) as s:
+        plain = s.serve(trace)
+    print("serving: back-to-back plain runs done", file=sys.stderr, flush=True)
+    with sv.Server(be, sv.default_budget(), slots=args.serving_slots, reuse=True) as s:
+        served = s.serve(trace)
+        pool = {"boxes_created": s.created, "boxes_evicted": s.evicted, "peak_running": s.peak_running,
+                "peak_device_bytes": s.peak_in_use}
+    same = all(a.status == b.status == "ok" and a.result["cycles"] == b.result["cycles"]
+               and np.array_equal(a.result["pk"], b.result["pk"]) and a.result["phi_sum"] == b.result["phi_sum"]
+               for a, b in zip(plain, served))
+    ps, pp = sv.summarize(served), sv.summarize(plain)
+    svc = {}
+    for o in plain:
+        r = trace[o.rid]
+        svc.setdefault("%d_%s" % (r.ngrid, "gr" if r.gr else "newton"), []).append(o.service)
+    print("serving: p99 %.3f s served, %.3f s plain" % (ps["p99_s"], pp["p99_s"]), file=sys.stderr, flush=True)
+    return {"trace": {"requests": len(trace), "rate_per_s": args.serving_rate, "seed": 2024,
+                      "grids": [64, 96, 128, 192, 256], "weights": [0.3, 0.25, 0.2, 0.15, 0.1],
+                      "gr_fraction": 0.5, "z": "100 -> 0", "form": "fp64"},
+            "slots": args.serving_slots, "served": ps, "back_to_back_plain": pp,
+            "p99_speedup_vs_plain": round(pp["p99_s"] / ps["p99_s"], 3) if ps["p99_s"] else None,
+            "throughput_ratio_vs_plain": (round(ps["particle_steps_per_s"] / pp["particle_steps_per_s"], 3)
+                                          if ps["particle_steps_per_s"] and pp["particle_steps_per_s"] else None),
+            "identical_results_alone_and_in_the_mix": bool(same),
+            "plain_service_s_mean": {k: round(float(np.mean(v)), 4) for k, v in sorted(svc.items())},
+            "pool": pool}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,6 +398,12 @@ def main():
                     help="skip the FP32 form's timing and error report")
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the whole run from z = 100 to z = 0 (both forms)")
+    ap.add_argument("--serving", action="store_true",
+                    help="also serve a Poisson trace of whole simulations against back-to-back plain runs "
+                         "(off by default: its first GPU run ended in a fault whose cause is not found)")
+    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-rate", type=float, default=6.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 1 GiB device copies (scale for the PMC byte counters)")
     args = ap.parse_args()
@@ -468,6 +522,8 @@ def main():
     if box32 is not None:
         box32.close()
     box.close()
+    if args.serving:
+        out["serving"] = serving_report(args)
     print(json.dumps(out))
 
 

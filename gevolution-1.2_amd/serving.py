@@ -1,0 +1,342 @@
+"""serving.py -- whole simulations served concurrently on one GPU.
+
+A request is one run of the reference's program: generateIC_basic
+(main.cpp:300; ic_basic.hpp:1626-2259) from a BBKS table for its cosmology
+and seed, the main loop (main.cpp:372-879) until z = 0, then the matter
+power spectrum of output.hpp:1990-2028.  gevolution itself runs one
+simulation per process (main.cpp:115-149); this loop has no counterpart
+there and calls only the C ABI (include/gev_pm.h) through gevpm.
+
+Admission.  Requests are taken in arrival order.  The head of the queue is
+admitted when a slot is free and its box's device footprint
+(gev_ctx_footprint_bytes) fits in the memory budget next to the running
+boxes and the idle pooled ones; idle boxes are evicted, least recently used
+first, to make room.  A request larger than the whole budget is rejected
+at once rather than left to block the queue.
+
+Residency and slot reuse.  A finished box goes back to a pool keyed by
+(grid, form), and the next request of that shape reuses it: the initial
+conditions overwrite the whole state, so nothing of the previous run
+survives, and the allocation is saved.
+
+Overlap.  Each admitted request runs in a host thread on its own box, and
+so on its own HIP stream; ctypes releases the interpreter lock inside
+library calls, so the streams of different requests overlap on the device.
+A request's results do not depend on what else runs: its kernels touch only
+its own box, in a fixed order, and every atomic in them either only counts
+(the filing, whose order a sort then fixes) or is order-independent
+(integer maxima, fixed-point sums).
+"""
+import collections
+import math
+import os
+import threading
+import time
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import gevpm
+
+C_LIGHT = 2997.92458  # 100 km/s units (metadata.hpp:98)
+
+
+@dataclass
+class Request:
+    """One simulation: grid N with N^3 particles, box in Mpc/h, to z = 0."""
+    rid: int
+    ngrid: int
+    gr: bool = True
+    Omega_m: float = 0.31
+    A_s: float = 2.1e-9
+    seed: int = 0
+    boxsize: float = 0.0          # Mpc/h; 0 means 5 Mpc/h per cell
+    arrival: float = 0.0          # seconds after the start of the trace
+    form: int = gevpm.FORM_FP64
+    Omega_b: float = 0.05
+    h: float = 0.67
+    n_s: float = 0.965
+    z_in: float = 100.0
+    z_end: float = 0.0
+
+    @property
+    def box_mpc(self):
+        return self.boxsize if self.boxsize > 0 else 5.0 * self.ngrid
+
+    @property
+    def npart(self):
+        return self.ngrid ** 3
+
+
+@dataclass
+class Outcome:
+    rid: int
+    status: str                   # "ok", "rejected" or "failed"
+    arrival: float
+    admitted: float = math.nan
+    finished: float = math.nan
+    reused: bool = False
+    error: str = ""
+    result: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        """Arrival to completion, seconds."""
+        return self.finished - self.arrival
+
+    @property
+    def service(self):
+        return self.finished - self.admitted
+
+
+def percentile(values, p):
+    """Nearest-rank percentile (p in (0, 100])."""
+    v = sorted(values)
+    if not v:
+        return math.nan
+    return v[max(0, math.ceil(p / 100.0 * len(v)) - 1)]
+
+
+def poisson_trace(n, rate, seed, grids=(64, 96, 128, 192, 256), weights=(0.3, 0.25, 0.2, 0.15, 0.1),
+                  gr_fraction=0.5, Omega_m=(0.25, 0.35), A_s=(1.8e-9, 2.4e-9)):
+    """n requests with exponential inter-arrival times (rate per second),
+    grids drawn with the given weights, GR or Newtonian with probability
+    gr_fraction, Omega_m and A_s uniform in the given ranges; request i
+    has seed i."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
+    g = rng.choice(np.asarray(grids), size=n, p=np.asarray(weights) / np.sum(weights))
+    gr = rng.random(n) < gr_fraction
+    om = rng.uniform(*Omega_m, n)
+    a_s = rng.uniform(*A_s, n)
+    return [Request(i, int(g[i]), bool(gr[i]), float(om[i]), float(a_s[i]), i, arrival=float(t[i]))
+            for i in range(n)]
+
+
+class Server:
+    """Admission, slot reuse and overlap over a backend with
+    footprint(req), key(req), create(req), run(box, req) -> dict and
+    destroy(box)."""
+
+    def __init__(self, backend, memory_budget, slots=4, reuse=True, clock=time.perf_counter):
+        if slots < 1:
+            raise ValueError("slots must be at least 1")
+        self.backend = backend
+        self.budget = int(memory_budget)
+        self.slots = int(slots)
+        self.reuse = bool(reuse)
+        self.clock = clock
+        self.in_use = 0                              # bytes of running and pooled boxes
+        self._pool = collections.OrderedDict()       # (key, serial) -> (box, bytes), LRU first
+        self._serial = 0
+        self.peak_in_use = 0
+        self.peak_running = 0
+        self.created = 0
+        self.evicted = 0
+
+    # -- memory and pool (dispatcher thread only)
+    def _take(self, req, need):
+        """A box for req, or None while the memory is held by running boxes."""
+        key = self.backend.key(req)
+        if self.reuse:
+            for pk in self._pool:
+                if pk[0] == key:
+                    box, _ = self._pool.pop(pk)
+                    return box, True
+        while self.in_use + need > self.budget and self._pool:
+            _, (box, nb) = self._pool.popitem(last=False)
+            self.backend.destroy(box)
+            self.in_use -= nb
+            self.evicted += 1
+        if self.in_use + need > self.budget:
+            return None
+        box = self.backend.create(req)
+        self.created += 1
+        self.in_use += need
+        self.peak_in_use = max(self.peak_in_use, self.in_use)
+        return box, False
+
+    def _give_back(self, req, box, need, healthy):
+        if self.reuse and healthy:
+            self._serial += 1
+            self._pool[(self.backend.key(req), self._serial)] = (box, need)
+        else:
+            self.backend.destroy(box)
+            self.in_use -= need
+
+    def close(self):
+        """Frees the pooled boxes."""
+        while self._pool:
+            _, (box, nb) = self._pool.popitem(last=False)
+            self.backend.destroy(box)
+            self.in_use -= nb
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the loop
+    def serve(self, requests):
+        """Runs the requests as they arrive; returns their outcomes in
+        request order.  Arrival times are seconds after the call."""
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        outcomes = {}
+        waiting = collections.deque()
+        running = {}
+        t0 = self.clock()
+        i = 0
+        with ThreadPoolExecutor(max_workers=self.slots) as ex:
+            while i < len(reqs) or waiting or running:
+                now = self.clock() - t0
+                while i < len(reqs) and reqs[i].arrival <= now:
+                    waiting.append(reqs[i])
+                    i += 1
+                while waiting and len(running) < self.slots:
+                    r = waiting[0]
+                    need = int(self.backend.footprint(r))
+                    if need < 0 or need > self.budget:
+                        waiting.popleft()
+                        outcomes[r.rid] = Outcome(r.rid, "rejected", r.arrival,
+                                                  error="footprint %d B exceeds the budget %d B" % (need, self.budget))
+                        continue
+                    try:
+                        got = self._take(r, need)
+                    except Exception as e:  # allocation failed: report it, keep serving
+                        waiting.popleft()
+                        outcomes[r.rid] = Outcome(r.rid, "failed", r.arrival, error="create: %s" % e)
+                        continue
+                    if got is None:
+                        break  # the head waits for memory; arrival order is kept
+                    waiting.popleft()
+                    box, reused = got
+                    adm = self.clock() - t0
+                    running[ex.submit(self.backend.run, box, r)] = (r, box, need, adm, reused)
+                    self.peak_running = max(self.peak_running, len(running))
+                timeout = None
+                if i < len(reqs):
+                    timeout = max(0.0, reqs[i].arrival - (self.clock() - t0))
+                if running:
+                    done, _ = wait(list(running), timeout=timeout, return_when=FIRST_COMPLETED)
+                else:
+                    if timeout:
+                        time.sleep(timeout)
+                    done = ()
+                for f in done:
+                    r, box, need, adm, reused = running.pop(f)
+                    fin = self.clock() - t0
+                    try:
+                        res = f.result()
+                        outcomes[r.rid] = Outcome(r.rid, "ok", r.arrival, adm, fin, reused, result=res)
+                        self._give_back(r, box, need, True)
+                    except Exception as e:
+                        outcomes[r.rid] = Outcome(r.rid, "failed", r.arrival, adm, fin, reused, error=str(e))
+                        self._give_back(r, box, need, False)
+        return [outcomes[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+
+def summarize(outcomes):
+    """p50 / p99 time to completion (arrival to z = 0), makespan and the
+    aggregate particle-steps/s over the trace."""
+    ok = [o for o in outcomes if o.status == "ok"]
+    lat = [o.latency for o in ok]
+    steps = sum(o.result.get("particle_steps", 0) for o in ok)
+    t_first = min((o.arrival for o in outcomes), default=0.0)
+    t_last = max((o.finished for o in ok), default=0.0)
+    span = t_last - t_first
+    return {"completed": len(ok), "rejected": sum(o.status == "rejected" for o in outcomes),
+            "failed": sum(o.status == "failed" for o in outcomes),
+            "p50_s": round(percentile(lat, 50), 4), "p99_s": round(percentile(lat, 99), 4),
+            "makespan_s": round(span, 4), "particle_steps": int(steps),
+            "particle_steps_per_s": round(steps / span, 1) if span > 0 else None}
+
+
+class GevBackend:
+    """Runs requests on the current GPU through the C ABI: one box per slot,
+    each on its own stream (gev_ctx_create with a NULL stream)."""
+
+    def __init__(self, template=None, numbins=None, kgrid=None):
+        import bbks
+        self._bbks = bbks
+        if template is None:
+            # the sc1 crystal's particles, a 4^3 lattice at (i + 1/2) / 4, x
+            # fastest (the template file lists them in another order: pass it,
+            # read with gevpm.load_homogeneous_template, for the reference's IDs)
+            g = (np.arange(4) + 0.5) / 4.0
+            z, y, x = np.meshgrid(g, g, g, indexing="ij")
+            template = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+        self.template = np.ascontiguousarray(template, dtype=np.float32).reshape(-1, 3)
+        self.tile = round(len(self.template) ** (1.0 / 3.0))
+        if self.tile ** 3 != len(self.template):
+            raise ValueError("the template must hold a cube number of particles")
+        self.numbins = numbins
+        self.kgrid = np.logspace(-5.0, np.log10(300.0), 400) if kgrid is None else np.asarray(kgrid)
+        self._lock = threading.Lock()
+
+    def key(self, req):
+        return (req.ngrid, req.form)
+
+    def footprint(self, req):
+        if req.ngrid % self.tile:
+            return -1
+        return gevpm.footprint_bytes(req.ngrid, req.npart, req.form)
+
+    def create(self, req):
+        return gevpm.Box(req.ngrid, req.npart, form=req.form)
+
+    def destroy(self, box):
+        box.close()
+
+    @staticmethod
+    def setup(req):
+        """(sim, state) of main.cpp:278-295 for the request."""
+        cosmo = gevpm.make_cosmology(req.Omega_m - req.Omega_b, req.Omega_b, req.h)
+        fourpiG = 1.5 * req.box_mpc ** 2 / C_LIGHT ** 2
+        sim = gevpm.SimParams(1 if req.gr else 0, 0, 48.0, 0.04, 100.0, fourpiG, cosmo)
+        a0 = 1.0 / (1.0 + req.z_in)
+        st = gevpm.CycleState(a0, 0.0, gevpm.next_dtau(sim, req.ngrid, a0), 0.0, 0.0, 0.0, 0.0, 0, 0)
+        return sim, st
+
+    @staticmethod
+    def cycles_to(sim, st, ngrid, z_end):
+        """Cycles until z <= z_end, from the host's own recurrence for a and
+        dtau (two half RK4 steps per cycle, main.cpp:792,814,869-872), the
+        one gev_run_cycles follows."""
+        a, dtau, n = st.a, st.dtau, 0
+        fourpiG, cosmo = sim.fourpiG, sim.cosmo
+        while 1.0 / a - 1.0 > z_end + 1e-12:
+            a = gevpm.rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau)
+            a = gevpm.rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau)
+            dtau = gevpm.next_dtau(sim, ngrid, a)
+            n += 1
+        return n
+
+    def run(self, box, req):
+        N = req.ngrid
+        sim, st = self.setup(req)
+        L, h = req.box_mpc, req.h
+        k, d, t = self._bbks.bbks_table(self.kgrid, req.z_in, req.Omega_m, h)
+        ic = gevpm.ICParams(int(req.seed), 1, 1, N // self.tile, req.z_in, req.A_s, req.n_s, 0.05, L)
+        mv, _ = box.generateIC_basic(sim, ic, self.template, k * L, d, t * L / h, d, t * L / h)
+        st.maxvel = mv
+        n = self.cycles_to(sim, st, N, req.z_end)
+        if n > 0:
+            box.run_cycles(sim, st, n)
+        # the delta spectrum of output.hpp:1990-2028 (Newtonian runs: the CIC mass)
+        box.projection_init(gevpm.SOURCE)
+        if req.gr:
+            box.projection_T00_project(st.a, True, 1.0)
+        else:
+            box.scalarProjectionCIC_project(1.0)
+        box.fft(gevpm.PLAN_SOURCE, gevpm.FFT_FORWARD)
+        kb, pk, _, _, occ = box.extractPowerSpectrum(gevpm.SCALARFT, self.numbins or N // 2)
+        phi_sum = box.field_sum(gevpm.PHI)
+        return {"a": st.a, "cycles": st.cycle, "particle_steps": req.npart * st.cycle,
+                "k": kb, "pk": pk, "occupation": occ, "phi_sum": phi_sum, "maxvel": st.maxvel}
+
+
+def default_budget():
+    """288 GB of HBM less 8 GB for the runtime and other processes."""
+    return int(os.environ.get("GEV_SERVING_BUDGET", 280 * 10 ** 9))

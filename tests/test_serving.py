@@ -1,0 +1,204 @@
+"""Serving loop on the CPU: admission by footprint, slot reuse, eviction,
+rejection and failure handling, with a fake backend standing in for the
+device (the GPU tests run the real one)."""
+import math
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "gevolution-1.2_amd"))
+
+import serving as sv  # noqa: E402
+
+
+class FakeBackend:
+    """Boxes cost N^3 bytes; a run sleeps in proportion to N and returns
+    which box it ran on."""
+
+    def __init__(self, unit=0.004, fail=()):
+        self.unit = unit
+        self.fail = set(fail)
+        self.lock = threading.Lock()
+        self.live = 0
+        self.peak_live = 0
+        self.boxes = set()
+        self.next_id = 0
+
+    def key(self, r):
+        return (r.ngrid, r.form)
+
+    def footprint(self, r):
+        return r.ngrid ** 3
+
+    def create(self, r):
+        with self.lock:
+            self.next_id += 1
+            box = (self.next_id, r.ngrid)
+            self.boxes.add(box)
+        return box
+
+    def destroy(self, box):
+        with self.lock:
+            self.boxes.remove(box)
+
+    def run(self, box, r):
+        assert box in self.boxes and box[1] == r.ngrid
+        with self.lock:
+            self.live += 1
+            self.peak_live = max(self.peak_live, self.live)
+        time.sleep(self.unit * r.ngrid / 16)
+        with self.lock:
+            self.live -= 1
+        if r.rid in self.fail:
+            raise RuntimeError("injected failure")
+        return {"particle_steps": r.ngrid ** 3 * 10, "box": box}
+
+
+def reqs(grids, arrivals=None):
+    arrivals = arrivals or [0.0] * len(grids)
+    return [sv.Request(i, g, arrival=t) for i, (g, t) in enumerate(zip(grids, arrivals))]
+
+
+def test_all_complete_within_slots_and_budget():
+    be = FakeBackend()
+    budget = 3 * 32 ** 3
+    with sv.Server(be, budget, slots=3) as s:
+        out = s.serve(reqs([16, 32, 16, 32, 16, 32, 16, 16]))
+        assert [o.rid for o in out] == list(range(8))
+        assert all(o.status == "ok" for o in out)
+        assert s.peak_running <= 3 and be.peak_live <= 3
+        assert s.peak_in_use <= budget
+        assert all(o.admitted >= o.arrival and o.finished >= o.admitted for o in out)
+    assert s.in_use == 0 and not be.boxes  # close() freed the pool
+
+
+def test_slot_reuse_keeps_one_box_per_shape():
+    be = FakeBackend()
+    with sv.Server(be, 10 ** 9, slots=1) as s:
+        out = s.serve(reqs([32, 32, 32, 32]))
+        assert s.created == 1
+        assert [o.reused for o in out] == [False, True, True, True]
+        assert len({o.result["box"] for o in out}) == 1
+    with sv.Server(be, 10 ** 9, slots=1, reuse=False) as s:
+        out = s.serve(reqs([32, 32, 32]))
+        assert s.created == 3 and not any(o.reused for o in out)
+        assert not be.boxes  # plain runs free their box at once
+
+
+def test_idle_boxes_are_evicted_to_admit_another_shape():
+    be = FakeBackend()
+    budget = 32 ** 3 + 16 ** 3 - 1  # a 32 box and a 16 box never fit together
+    with sv.Server(be, budget, slots=2) as s:
+        out = s.serve(reqs([16, 32, 16]))
+        assert all(o.status == "ok" for o in out)
+        assert s.evicted >= 1 and s.peak_in_use <= budget
+        assert be.peak_live == 1
+
+
+def test_oversized_request_is_rejected_without_blocking_the_queue():
+    be = FakeBackend()
+    with sv.Server(be, 32 ** 3, slots=2) as s:
+        out = s.serve(reqs([16, 64, 32, 16]))
+    assert [o.status for o in out] == ["ok", "rejected", "ok", "ok"]
+    assert "exceeds the budget" in out[1].error
+
+
+def test_failure_is_reported_and_its_box_not_reused():
+    be = FakeBackend(fail={1})
+    with sv.Server(be, 10 ** 9, slots=1) as s:
+        out = s.serve(reqs([16, 16, 16]))
+        assert [o.status for o in out] == ["ok", "failed", "ok"]
+        assert "injected" in out[1].error
+        assert out[2].result["box"] != out[1].result.get("box")
+        assert s.created == 2  # the failed run's box was destroyed, not pooled
+    assert s.in_use == 0 and not be.boxes
+
+
+def test_arrivals_are_waited_for_and_admitted_in_order():
+    be = FakeBackend(unit=0.002)
+    arr = [0.0, 0.05, 0.06, 0.2]
+    with sv.Server(be, 10 ** 9, slots=1) as s:
+        out = s.serve(reqs([16, 16, 16, 16], arr))
+    for o, t in zip(out, arr):
+        assert o.admitted >= t - 1e-9
+    adm = [o.admitted for o in out]
+    assert adm == sorted(adm)
+    summ = sv.summarize(out)
+    assert summ["completed"] == 4 and summ["makespan_s"] >= 0.2
+    assert summ["particle_steps"] == 4 * 16 ** 3 * 10
+
+
+def test_overlap_beats_back_to_back_on_a_loaded_trace():
+    trace = reqs([32] * 8)
+    t0 = time.perf_counter()
+    with sv.Server(FakeBackend(unit=0.01), 10 ** 9, slots=4) as s:
+        mix = sv.summarize(s.serve(trace))
+    t_mix = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with sv.Server(FakeBackend(unit=0.01), 10 ** 9, slots=1, reuse=False) as s:
+        plain = sv.summarize(s.serve(trace))
+    t_plain = time.perf_counter() - t0
+    assert t_mix < t_plain
+    assert mix["p99_s"] < plain["p99_s"]
+
+
+def test_percentile_nearest_rank():
+    v = list(range(1, 101))
+    assert sv.percentile(v, 50) == 50 and sv.percentile(v, 99) == 99 and sv.percentile(v, 100) == 100
+    assert sv.percentile([3.0], 99) == 3.0
+    assert math.isnan(sv.percentile([], 50))
+    assert sv.percentile([5, 1, 4, 2, 3], 50) == 3
+
+
+def test_poisson_trace():
+    a = sv.poisson_trace(4000, 8.0, 3)
+    b = sv.poisson_trace(4000, 8.0, 3)
+    assert [(r.ngrid, r.gr, r.arrival) for r in a] == [(r.ngrid, r.gr, r.arrival) for r in b]
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 1 / 8.0) < 0.1 / 8.0
+    grids = np.array([r.ngrid for r in a])
+    for g, w in zip((64, 96, 128, 192, 256), (0.3, 0.25, 0.2, 0.15, 0.1)):
+        assert abs(np.mean(grids == g) - w) < 0.03
+    assert abs(np.mean([r.gr for r in a]) - 0.5) < 0.03
+    om = np.array([r.Omega_m for r in a])
+    assert om.min() >= 0.25 and om.max() <= 0.35
+    assert [r.seed for r in a[:5]] == [0, 1, 2, 3, 4]
+    assert a[0].box_mpc == 5.0 * a[0].ngrid
+
+
+def test_cycles_to_z0_follow_the_cycle_recurrence():
+    """The host count of cycles to z = 0 for the 512^3, 2048 Mpc/h run is
+    the 117 cycles the device run takes, and it stops at the first cycle
+    that reaches z <= 0."""
+    gev = pytest.importorskip("gevpm")
+    if not os.path.exists(gev.LIB_PATH):
+        pytest.skip("library not built")
+    r = sv.Request(0, 512, boxsize=2048.0)
+    sim, st = sv.GevBackend.setup(r)
+    n = sv.GevBackend.cycles_to(sim, st, 512, 0.0)
+    assert n == 117
+    a, dtau = st.a, st.dtau
+    for i in range(n):
+        assert 1.0 / a - 1.0 > 0.0
+        a = gev.rungekutta4bg(a, sim.fourpiG, sim.cosmo, 0.5 * dtau)
+        a = gev.rungekutta4bg(a, sim.fourpiG, sim.cosmo, 0.5 * dtau)
+        dtau = gev.next_dtau(sim, 512, a)
+    assert 1.0 / a - 1.0 <= 0.0
+
+
+def test_default_template_is_the_sc1_crystal():
+    gev = pytest.importorskip("gevpm")
+    if not os.path.exists(gev.LIB_PATH):
+        pytest.skip("library not built")
+    tpl = gev.load_homogeneous_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
+    be = sv.GevBackend()
+    assert be.tile == 4
+    # the same particles (the file lists them in another order, which only
+    # changes which particle gets which ID)
+    key = lambda t: t[np.lexsort(t.T[::-1])]
+    np.testing.assert_array_equal(key(np.asarray(tpl, dtype=np.float32).reshape(-1, 3)), key(be.template))
