@@ -202,3 +202,65 @@ def test_default_template_is_the_sc1_crystal():
     # changes which particle gets which ID)
     key = lambda t: t[np.lexsort(t.T[::-1])]
     np.testing.assert_array_equal(key(np.asarray(tpl, dtype=np.float32).reshape(-1, 3)), key(be.template))
+
+
+class RecordingBox:
+    """Stands in for gevpm.Box: records the calls GevBackend.run makes."""
+
+    def __init__(self):
+        self.calls = []
+
+    def generateIC_basic(self, sim, ic, tpl, k, dt, tt, dc, tc):
+        self.calls.append(("ic", sim.gr_flag, ic.seed, ic.numtile, ic.boxsize, ic.A_s, len(tpl), float(k[0])))
+        return 1e-3, 0.1
+
+    def run_cycles(self, sim, st, n):
+        self.calls.append(("cycles", n))
+        st.cycle += n
+        st.a = 1.0
+
+    def projection_init(self, f):
+        self.calls.append(("init", f))
+
+    def projection_T00_project(self, a, use_phi, coeff):
+        self.calls.append(("T00", a, use_phi))
+
+    def scalarProjectionCIC_project(self, coeff):
+        self.calls.append(("CIC",))
+
+    def fft(self, plan, direction):
+        self.calls.append(("fft", plan, direction))
+
+    def extractPowerSpectrum(self, f, numbins):
+        self.calls.append(("pk", f, numbins))
+        z = np.zeros(numbins)
+        return z, z + 1.0, z, z, np.ones(numbins, dtype=np.int32)
+
+    def field_sum(self, f):
+        return 0.5
+
+
+@pytest.mark.parametrize("gr", [True, False])
+def test_backend_runs_the_reference_sequence(gr):
+    """IC generation with the request's seed, tiling N/4 and a 5 Mpc/h-per-cell
+    box; the host-counted cycles to z = 0 in one call; then the delta
+    spectrum (T00 with phi for GR, the CIC mass for Newtonian runs) in N/2
+    bins."""
+    gev = pytest.importorskip("gevpm")
+    if not os.path.exists(gev.LIB_PATH):
+        pytest.skip("library not built")
+    r = sv.Request(3, 32, gr=gr, Omega_m=0.3, A_s=2.0e-9, seed=11)
+    sim, st = sv.GevBackend.setup(r)
+    n = sv.GevBackend.cycles_to(sim, st, 32, 0.0)
+    box = RecordingBox()
+    res = sv.GevBackend().run(box, r)
+    ic = box.calls[0]
+    assert ic[:6] == ("ic", 1 if gr else 0, 11, 8, 160.0, 2.0e-9) and ic[6] == 64
+    assert abs(ic[7] - 1e-5 * 160.0) < 1e-18  # k in units of 1/box (loadTransferFunctions)
+    assert box.calls[1] == ("cycles", n)
+    proj = ("T00", 1.0, True) if gr else ("CIC",)
+    assert box.calls[2:] == [("init", gev.SOURCE), proj, ("fft", gev.PLAN_SOURCE, gev.FFT_FORWARD),
+                             ("pk", gev.SCALARFT, 16)]
+    assert res["cycles"] == n and res["particle_steps"] == 32 ** 3 * n
+    assert sim.fourpiG == 1.5 * 160.0 ** 2 / sv.C_LIGHT ** 2
+    assert abs(sim.cosmo.Omega_cdm - 0.25) < 1e-15 and sim.cosmo.Omega_b == 0.05
