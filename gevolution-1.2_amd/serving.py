@@ -209,6 +209,12 @@ class Server:
                         outcomes[r.rid] = Outcome(r.rid, "failed", r.arrival, error="create: %s" % e)
                         continue
                     if got is None:
+                        if not running:  # nothing will free memory: never wait forever
+                            waiting.popleft()
+                            outcomes[r.rid] = Outcome(r.rid, "rejected", r.arrival,
+                                                      error="footprint %d B does not fit next to %d B in use"
+                                                      % (need, self.in_use))
+                            continue
                         break  # the head waits for memory; arrival order is kept
                     waiting.popleft()
                     box, reused = got

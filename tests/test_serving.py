@@ -264,3 +264,12 @@ def test_backend_runs_the_reference_sequence(gr):
     assert res["cycles"] == n and res["particle_steps"] == 32 ** 3 * n
     assert sim.fourpiG == 1.5 * 160.0 ** 2 / sv.C_LIGHT ** 2
     assert abs(sim.cosmo.Omega_cdm - 0.25) < 1e-15 and sim.cosmo.Omega_b == 0.05
+
+
+def test_a_request_that_cannot_fit_beside_held_memory_is_rejected_not_waited_for():
+    be = FakeBackend()
+    with sv.Server(be, 32 ** 3, slots=1) as s:
+        s.in_use = 32 ** 3  # memory held outside the pool (accounting the server cannot free)
+        out = s.serve(reqs([16]))
+        s.in_use = 0
+    assert out[0].status == "rejected" and "does not fit" in out[0].error
