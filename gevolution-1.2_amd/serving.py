@@ -30,7 +30,6 @@ its own box, in a fixed order, and every atomic in them either only counts
 import collections
 import math
 import os
-import threading
 import time
 from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
@@ -279,7 +278,6 @@ class GevBackend:
             raise ValueError("the template must hold a cube number of particles")
         self.numbins = numbins
         self.kgrid = np.logspace(-5.0, np.log10(300.0), 400) if kgrid is None else np.asarray(kgrid)
-        self._lock = threading.Lock()
 
     def key(self, req):
         return (req.ngrid, req.form)
@@ -344,5 +342,6 @@ class GevBackend:
 
 
 def default_budget():
-    """288 GB of HBM less 8 GB for the runtime and other processes."""
+    """Device bytes the server may hold: GEV_SERVING_BUDGET if set, else
+    288 GB of HBM less 8 GB for the runtime and other processes."""
     return int(os.environ.get("GEV_SERVING_BUDGET", 280 * 10 ** 9))
