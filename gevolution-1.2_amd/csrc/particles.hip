@@ -869,10 +869,16 @@ __global__ __launch_bounds__(TPB, (sizeof(F) == 4 ? GEV_KICK_WAVES_F32 : 1)) GEV
         jsn = sorted_index(j + TPB);
         pn = perm[jsn];
     }
-    // field staging: element k of this thread's share of field f's region
+    // field staging: element k of this thread's share of field f's region.
+    // A tile cut short by the box edge (nx < KTX, e.g. every tile of N < 32)
+    // only reads slots up to its own extent plus the upper halo, so the
+    // region is clamped there: the coordinate stays within [x0 - 1, x0 + nx]
+    // (x0 + nx <= N), which one periodic step maps into [0, N).  The slots
+    // beyond hold copies of the last staged cell and are never read.
     auto cell_of = [&](auto reg, int i) {
         using R = decltype(reg);
-        const int rx = i % R::SX, ry = (i / R::SX) % R::SY, rz = i / (R::SX * R::SY);
+        const int rx = min(i % R::SX, nx + R::OX), ry = min((i / R::SX) % R::SY, ny + R::OY);
+        const int rz = min(i / (R::SX * R::SY), nz + R::OZ);
         return L(N, wrapi(x0 - R::OX + rx, N), wrapi(y0 - R::OY + ry, N), wrapi(z0 - R::OZ + rz, N));
     };
     constexpr int PS = (Region<-1>::SIZE + TPB - 1) / TPB, P0 = (Region<0>::SIZE + TPB - 1) / TPB;

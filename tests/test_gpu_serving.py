@@ -10,11 +10,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "gevolution-1.2_amd"))
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the first GPU run of these tests (in one command with other new code) ended in a "
-                            "GPU memory fault whose cause has not been found; they stay off the GPU until it is"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def _requests():
