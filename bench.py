@@ -399,8 +399,7 @@ def main():
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the whole run from z = 100 to z = 0 (both forms)")
     ap.add_argument("--serving", action="store_true",
-                    help="also serve a Poisson trace of whole simulations against back-to-back plain runs "
-                         "(off by default: its first GPU run ended in a fault whose cause is not found)")
+                    help="also serve a Poisson trace of whole simulations against back-to-back plain runs")
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-rate", type=float, default=6.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serving-slots", type=int, default=4)
