@@ -3,6 +3,7 @@
 // kernel timers.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -637,14 +638,15 @@ GEV_EXPORT double gev_next_dtau(const gev_sim_params *p, int ngrid, double a) {
     return p->steplimit / hconf(a, p->fourpiG, &p->cosmo);
 }
 
-// One cycle of main.cpp:372-879 (one CDM species, no outputs).  Host scalars
-// are advanced exactly as the reference does; fields and particles never
-// leave the device.  T00hom, phi(k=0) and maxvel are left on the device and
-// read once by gev_run_cycles.
+// The field stage of one cycle of main.cpp:372-879 (one CDM species):
+// projections, both prepareFTsource stencils, the FFTs and k-space solves
+// (main.cpp:372-598).  Host scalars are read, not advanced; fields and
+// particles never leave the device.  T00hom and phi(k=0) are left on the
+// device.
 // rec: when not null, a pinned slot that receives (T00hom * N^3, phi(k=0)) of
 // this cycle, copied asynchronously at the point where main.cpp:520-537
 // records the background line
-static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, double *rec) {
+static int cycle_fields(gev_ctx *c, const gev_sim_params *p, const gev_cycle_state *st, double *rec) {
     const int N = c->N;
     const double dx = 1.0 / (double)N;
     const double fourpiG = p->fourpiG;
@@ -737,9 +739,18 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, d
         if (rc) return rc;
     }
     c->zeroed[GEV_PHI] = c->zeroed[GEV_CHI] = c->zeroed[GEV_BI] = false;
+    return 0;
+}
 
-    // particle update (main.cpp:766-822): kick with {a, a^2 N}, half RK4 step,
-    // drift with the advanced a, half RK4 step.  One fused particle pass.
+// The particle stage (main.cpp:766-874): kick with {a, a^2 N}, half RK4 step,
+// drift with the advanced a, half RK4 step (one fused particle pass), then
+// the time-step update.  maxvel stays on the device.
+static int cycle_particles(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st) {
+    const int N = c->N;
+    const double fourpiG = p->fourpiG;
+    const gev_cosmology *cs = &p->cosmo;
+    double a = st->a;
+    int rc;
     double pk[2] = {a, a * a * N};
     const double dtk = (st->dtau + st->dtau_old) / 2.;
     const int nfk = p->gr_flag > 0 ? (1. / a < p->z_relax + 1. ? 3 : 2) : 1;
@@ -759,6 +770,31 @@ static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, d
     return 0;
 }
 
+// One cycle of main.cpp:372-879 (no outputs).  Host scalars are advanced
+// exactly as the reference does.
+static int run_cycle(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, double *rec) {
+    int rc = cycle_fields(c, p, st, rec);
+    return rc ? rc : cycle_particles(c, p, st);
+}
+
+// T00hom, phi(k=0) and maxvel of the last cycle into *st (synchronous)
+static int read_cycle_scalars(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st) {
+    double h[2];
+    unsigned long long vb;
+    int rc = check_hip(c, hipStreamSynchronize(c->stream), "cycle sync");
+    if (!rc) rc = check_hip(c, hipMemcpy(h, c->red_d, 2 * sizeof(double), hipMemcpyDeviceToHost), "cycle scalars");
+    if (!rc) rc = check_hip(c, hipMemcpy(&vb, c->red_u, sizeof(vb), hipMemcpyDeviceToHost), "cycle vmax");
+    if (rc) return rc;
+    double mv;
+    memcpy(&mv, &vb, sizeof(mv));
+    mv = sqrt(mv);
+    if (p->gr_flag > 0) mv /= sqrt(mv * mv + 1.0);
+    st->maxvel = mv;
+    st->T00hom = p->gr_flag > 0 ? h[0] / (double)n3(c) : 0.0;
+    st->phi_k0 = h[1];
+    return 0;
+}
+
 GEV_EXPORT int gev_run_cycles(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st,
                               int ncycles) {
     if (!c || !p || !st || ncycles < 0) return GEV_EINVAL;
@@ -766,25 +802,142 @@ GEV_EXPORT int gev_run_cycles(gev_ctx *c, const gev_sim_params *p, gev_cycle_sta
         return fail(c, GEV_EINVAL, "bad simulation flags");
     if (!(st->a > 0.) || !(st->dtau > 0.)) return fail(c, GEV_EINVAL, "state needs a > 0, dtau > 0");
     for (int i = 0; i < ncycles; i++) {
-        int rc = run_cycle(c, p, st, nullptr);
+        int rc;
+        if (st->stage == 1) {  // the field stage of this cycle ran at an output stop
+            rc = cycle_particles(c, p, st);
+            st->stage = 0;
+        } else {
+            rc = run_cycle(c, p, st, nullptr);
+        }
         if (rc) return rc;
     }
-    if (ncycles > 0) {
-        double h[2];
-        unsigned long long vb;
-        int rc = check_hip(c, hipStreamSynchronize(c->stream), "cycle sync");
-        if (!rc) rc = check_hip(c, hipMemcpy(h, c->red_d, 2 * sizeof(double), hipMemcpyDeviceToHost), "cycle scalars");
-        if (!rc) rc = check_hip(c, hipMemcpy(&vb, c->red_u, sizeof(vb), hipMemcpyDeviceToHost), "cycle vmax");
-        if (rc) return rc;
-        double mv;
-        memcpy(&mv, &vb, sizeof(mv));
-        mv = sqrt(mv);
-        if (p->gr_flag > 0) mv /= sqrt(mv * mv + 1.0);
-        st->maxvel = mv;
-        st->T00hom = p->gr_flag > 0 ? h[0] / (double)n3(c) : 0.0;
-        st->phi_k0 = h[1];
+    return ncycles > 0 ? read_cycle_scalars(c, p, st) : 0;
+}
+
+// ---------------------------------------------------------------- outputs
+
+namespace {
+
+// A number as writePowerSpectrum prints it ("%e" in the table, "%f" for the
+// header redshift) and its interpolating pass reads it back (fscanf %le / %lf),
+// tools.hpp:283-301,318-332.
+double text_round_trip(const char *fmt, double v) {
+    char buf[64];
+    snprintf(buf, sizeof buf, fmt, v);
+    return strtod(buf, nullptr);
+}
+
+struct PkWriter {
+    int numbins;
+    double rescalek, rescalep;
+    std::vector<double> kb, pw, ks, ps, p2, pre, pre_z;
+    std::vector<int32_t> oc, have;
+};
+
+// writeSpectra's delta spectrum (output.hpp:1820-1827 Newton, 2001-2010 GR)
+// and writePowerSpectrum (tools.hpp:268-346, EXACT_OUTPUT_REDSHIFTS) of
+// output j at the current a, into table j of the outputs
+int write_spectrum(gev_ctx *c, const gev_sim_params *p, const gev_cycle_state *st, PkWriter &w, int j,
+                   double zt, double *k, double *pk, double *sk, double *spk, int32_t *occ,
+                   gev_pk_output_info *info) {
+    const int nb = w.numbins;
+    int rc = gev_projection_init(c, GEV_SOURCE);
+    if (!rc) rc = p->gr_flag > 0 ? gev_projection_T00_project(c, st->a, 1, 1.0)
+                                 : gev_scalar_projection_cic_project(c, 1.0);
+    if (!rc) rc = gev_fft_execute(c, GEV_PLAN_SOURCE, GEV_FFT_FORWARD);
+    if (!rc) rc = gev_extract_power_spectrum(c, GEV_SCALARFT, nb, 1, 1, w.kb.data(), w.pw.data(), w.ks.data(),
+                                             w.ps.data(), w.oc.data());
+    if (rc) return rc;
+    double aw = st->a;
+    for (int i = 0; i < nb; i++) w.p2[i] = w.pw[i] / w.rescalep;
+    if (1. / aw < zt + 1.) {
+        if (w.have[j]) {  // the file of the look-ahead write exists: interpolate
+            double weight = w.pre_z[j];
+            weight = (weight - zt) / (1. + weight - 1. / aw);
+            for (int i = 0; i < nb; i++)
+                if (w.oc[i] > 0) w.p2[i] = w.pre[(size_t)j * nb + i];
+            for (int i = 0; i < nb; i++) w.p2[i] = (1. - weight) * w.p2[i] + weight * w.pw[i] / w.rescalep;
+            aw = 1. / (zt + 1.);
+        }
+    } else {  // the look-ahead write: what the file holds afterwards
+        w.have[j] = 1;
+        w.pre_z[j] = text_round_trip("%f", (1. / aw) - 1.);
+        for (int i = 0; i < nb; i++) w.pre[(size_t)j * nb + i] = w.oc[i] > 0 ? text_round_trip("%e", w.p2[i]) : 0.;
     }
+    for (int i = 0; i < nb; i++) {
+        const size_t o = (size_t)j * nb + i;
+        k[o] = w.kb[i] / w.rescalek;
+        pk[o] = w.p2[i];
+        sk[o] = w.ks[i] / w.rescalek;
+        spk[o] = w.oc[i] > 0 ? w.ps[i] / w.rescalep / sqrt((double)w.oc[i]) : 0.;
+        occ[o] = w.oc[i];
+    }
+    info[j].z = (1. / aw) - 1.;
+    info[j].cycle = st->cycle;
+    info[j].interpolated = aw != st->a;
     return 0;
+}
+
+}  // namespace
+
+GEV_EXPORT int gev_run_to_outputs(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, int32_t num_pk,
+                                  const double *z_pk, int32_t numbins, double boxsize, int32_t max_cycles,
+                                  double *k, double *pk, double *sigma_k, double *sigma_pk, int32_t *occupation,
+                                  gev_pk_output_info *info) {
+    if (!c || !p || !st) return GEV_EINVAL;
+    if (p->gr_flag < 0 || p->gr_flag > 1 || p->vector_flag < 0 || p->vector_flag > 1)
+        return fail(c, GEV_EINVAL, "bad simulation flags");
+    if (!(st->a > 0.) || !(st->dtau > 0.)) return fail(c, GEV_EINVAL, "state needs a > 0, dtau > 0");
+    if (num_pk < 1 || !z_pk || numbins < 1 || numbins > 1024 || !(boxsize > 0.) || max_cycles < 0)
+        return fail(c, GEV_EINVAL, "outputs need num_pk >= 1, z_pk, numbins in [1, 1024], boxsize > 0");
+    if (!k || !pk || !sigma_k || !sigma_pk || !occupation || !info) return fail(c, GEV_EINVAL, "null output");
+    for (int j = 1; j < num_pk; j++)
+        if (!(z_pk[j] < z_pk[j - 1])) return fail(c, GEV_EINVAL, "z_pk must be strictly descending");
+    const gev_cosmology *cs = &p->cosmo;
+    const double n3d = (double)c->N * (double)c->N * (double)c->N;
+    // rescalep of the delta spectrum: output.hpp:2026 (GR, no ncdm), 1853 (Newton)
+    const double om = p->gr_flag > 0 ? cs->Omega_cdm + cs->Omega_b + 0.0 : cs->Omega_m;
+    PkWriter w;
+    w.numbins = numbins;
+    w.rescalek = boxsize;
+    w.rescalep = n3d * n3d * 2. * M_PI * M_PI * om * om;
+    for (auto *v : {&w.kb, &w.pw, &w.ks, &w.ps, &w.p2}) v->assign(numbins, 0.);
+    w.pre.assign((size_t)num_pk * numbins, 0.);
+    w.pre_z.assign(num_pk, 0.);
+    w.oc.assign(numbins, 0);
+    w.have.assign(num_pk, 0);
+    int pkcount = 0, ran = 0, rc = 0;
+    // a run stopped at an output resumes at that cycle's look-ahead check
+    bool resume = st->stage == 1;
+    while (ran < max_cycles) {
+        if (!resume) {
+            rc = cycle_fields(c, p, st, nullptr);
+            // main.cpp:639-657
+            if (!rc && pkcount < num_pk && 1. / st->a < z_pk[pkcount] + 1.) {
+                rc = write_spectrum(c, p, st, w, pkcount, z_pk[pkcount], k, pk, sigma_k, sigma_pk, occupation,
+                                    info);
+                pkcount++;
+            }
+        }
+        resume = false;
+        // main.cpp:659-677: the look-ahead of EXACT_OUTPUT_REDSHIFTS
+        double tmp = st->a;
+        rk4bg(tmp, p->fourpiG, cs, 0.5 * st->dtau);
+        rk4bg(tmp, p->fourpiG, cs, 0.5 * st->dtau);
+        if (!rc && pkcount < num_pk && 1. / tmp < z_pk[pkcount] + 1.)
+            rc = write_spectrum(c, p, st, w, pkcount, z_pk[pkcount], k, pk, sigma_k, sigma_pk, occupation, info);
+        if (rc) return rc;
+        st->stage = 1;
+        // main.cpp:685-693: the loop ends once every output is written
+        if (pkcount >= num_pk) break;
+        rc = cycle_particles(c, p, st);
+        if (rc) return rc;
+        st->stage = 0;
+        ran++;
+    }
+    rc = read_cycle_scalars(c, p, st);
+    if (rc) return rc;
+    return pkcount >= num_pk ? 0 : fail(c, GEV_ESTATE, "max_cycles ran before the last output");
 }
 
 GEV_EXPORT int gev_run_cycles_recorded(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, int ncycles,
@@ -793,6 +946,7 @@ GEV_EXPORT int gev_run_cycles_recorded(gev_ctx *c, const gev_sim_params *p, gev_
     if (p->gr_flag < 0 || p->gr_flag > 1 || p->vector_flag < 0 || p->vector_flag > 1)
         return fail(c, GEV_EINVAL, "bad simulation flags");
     if (!(st->a > 0.) || !(st->dtau > 0.)) return fail(c, GEV_EINVAL, "state needs a > 0, dtau > 0");
+    if (st->stage != 0) return fail(c, GEV_ESTATE, "a stopped cycle is completed by gev_run_cycles first");
     const double H0 = hconf(1., p->fourpiG, &p->cosmo);
     const double n3d = (double)n3(c);
     int done = 0;  // records whose device values have been read back

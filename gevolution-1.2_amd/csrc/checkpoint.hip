@@ -47,6 +47,7 @@ bool get(FILE *f, void *p, size_t bytes) { return fread(p, 1, bytes, f) == bytes
 GEV_EXPORT int gev_hibernate(gev_ctx *c, const gev_sim_params *p, const gev_cycle_state *st, const char *path) {
     if (!c) return GEV_EINVAL;
     if (!p || !st || !path) return gev::fail(c, GEV_EINVAL, "bad hibernate arguments");
+    if (st->stage != 0) return gev::fail(c, GEV_ESTATE, "a stopped cycle is completed by gev_run_cycles first");
     const int64_t n3 = gev::n3(c), n = c->n;
     const bool parabolic = p->vector_flag == 0;
     int rc;

@@ -47,7 +47,7 @@ EXPORTED = [
     "gev_initialize_particle_positions", "gev_displace_particles_ic", "gev_initialize_q_ic",
     "gev_generate_ic_basic", "gev_extract_cross_spectrum", "gev_extract_power_spectrum",
     "gev_write_power_spectrum", "gev_hibernate", "gev_restart", "gev_run_cycles_recorded",
-    "gev_write_background", "gev_copy_state",
+    "gev_write_background", "gev_copy_state", "gev_run_to_outputs",
 ]
 
 
@@ -81,10 +81,14 @@ class BackgroundRecord(ctypes.Structure):
                 ("hconf_ratio", _d), ("phi_k0", _d), ("T00hom", _d)]
 
 
+class PkOutputInfo(ctypes.Structure):
+    _fields_ = [("z", _d), ("cycle", ctypes.c_int32), ("interpolated", ctypes.c_int32)]
+
+
 class CycleState(ctypes.Structure):
     _fields_ = [("a", _d), ("tau", _d), ("dtau", _d), ("dtau_old", _d), ("maxvel", _d),
                 ("T00hom", _d), ("phi_k0", _d), ("cycle", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32)]
+                ("stage", ctypes.c_int32)]
 
 
 _lib = None
@@ -163,6 +167,8 @@ def lib():
                                                  _p, _p, _p, _p, _p]
         L.gev_write_power_spectrum.argtypes = [_p, _p, _p, _p, _p, ctypes.c_int32, _d, _d, ctypes.c_char_p,
                                                ctypes.c_char_p, _d, _d]
+        L.gev_run_to_outputs.argtypes = [_p, _p, _p, ctypes.c_int32, _p, ctypes.c_int32, _d, ctypes.c_int32,
+                                         _p, _p, _p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -446,6 +452,25 @@ class Box:
         self._chk(lib().gev_run_cycles(self._c, ctypes.byref(sim), ctypes.byref(state), ncycles),
                   "run_cycles")
         return state
+
+    def run_to_outputs(self, sim, state, z_pk, numbins, boxsize, max_cycles=100000):
+        """The main loop with the delta power spectrum written at each z_pk
+        (descending), EXACT_OUTPUT_REDSHIFTS on (gev_run_to_outputs); returns
+        a dict of [len(z_pk), numbins] tables (k, pk, sigma_k, sigma_pk,
+        occupation) and per-output z, cycle and interpolated flags."""
+        z = np.ascontiguousarray(z_pk, dtype=np.float64)
+        m = len(z)
+        out = {n: np.zeros((m, numbins)) for n in ("k", "pk", "sigma_k", "sigma_pk")}
+        out["occupation"] = np.zeros((m, numbins), dtype=np.int32)
+        info = (PkOutputInfo * m)()
+        self._chk(lib().gev_run_to_outputs(self._c, ctypes.byref(sim), ctypes.byref(state), m, _ptr(z), numbins,
+                                           float(boxsize), int(max_cycles),
+                                           *[_ptr(out[n]) for n in ("k", "pk", "sigma_k", "sigma_pk", "occupation")],
+                                           info), "run_to_outputs")
+        out["z"] = np.array([i.z for i in info])
+        out["cycle"] = np.array([i.cycle for i in info], dtype=np.int32)
+        out["interpolated"] = np.array([i.interpolated for i in info], dtype=bool)
+        return out
 
     # -- timing
     def timing(self, on=True):

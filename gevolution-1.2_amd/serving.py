@@ -2,8 +2,11 @@
 
 A request is one run of the reference's program: generateIC_basic
 (main.cpp:300; ic_basic.hpp:1626-2259) from a BBKS table for its cosmology
-and seed, the main loop (main.cpp:372-879) until z = 0, then the matter
-power spectrum of output.hpp:1990-2028.  gevolution itself runs one
+and seed, then the main loop (main.cpp:372-879) with the delta power
+spectrum written at z_end (output.hpp:1796-1853, 1983-2028) the way the
+reference writes it with EXACT_OUTPUT_REDSHIFTS: interpolated in redshift
+between the two cycles around z_end, the loop ending after the field stage
+of the crossing cycle (gev_run_to_outputs).  gevolution itself runs one
 simulation per process (main.cpp:115-149); this loop has no counterpart
 there and calls only the C ABI (include/gev_pm.h) through gevpm.
 
@@ -143,13 +146,14 @@ class Server:
                 if pk[0] == key:
                     box, _ = self._pool.pop(pk)
                     return box, True
-        while self.in_use + need > self.budget and self._pool:
+        pooled = sum(nb for _, nb in self._pool.values())
+        if self.in_use - pooled + need > self.budget:
+            return None  # running boxes hold the memory: keep the pool for later requests
+        while self.in_use + need > self.budget:
             _, (box, nb) = self._pool.popitem(last=False)
             self.backend.destroy(box)
             self.in_use -= nb
             self.evicted += 1
-        if self.in_use + need > self.budget:
-            return None
         box = self.backend.create(req)
         self.created += 1
         self.in_use += need
@@ -303,20 +307,6 @@ class GevBackend:
         st = gevpm.CycleState(a0, 0.0, gevpm.next_dtau(sim, req.ngrid, a0), 0.0, 0.0, 0.0, 0.0, 0, 0)
         return sim, st
 
-    @staticmethod
-    def cycles_to(sim, st, ngrid, z_end):
-        """Cycles until z <= z_end, from the host's own recurrence for a and
-        dtau (two half RK4 steps per cycle, main.cpp:792,814,869-872), the
-        one gev_run_cycles follows."""
-        a, dtau, n = st.a, st.dtau, 0
-        fourpiG, cosmo = sim.fourpiG, sim.cosmo
-        while 1.0 / a - 1.0 > z_end + 1e-12:
-            a = gevpm.rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau)
-            a = gevpm.rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau)
-            dtau = gevpm.next_dtau(sim, ngrid, a)
-            n += 1
-        return n
-
     def run(self, box, req):
         N = req.ngrid
         sim, st = self.setup(req)
@@ -325,20 +315,12 @@ class GevBackend:
         ic = gevpm.ICParams(int(req.seed), 1, 1, N // self.tile, req.z_in, req.A_s, req.n_s, 0.05, L)
         mv, _ = box.generateIC_basic(sim, ic, self.template, k * L, d, t * L / h, d, t * L / h)
         st.maxvel = mv
-        n = self.cycles_to(sim, st, N, req.z_end)
-        if n > 0:
-            box.run_cycles(sim, st, n)
-        # the delta spectrum of output.hpp:1990-2028 (Newtonian runs: the CIC mass)
-        box.projection_init(gevpm.SOURCE)
-        if req.gr:
-            box.projection_T00_project(st.a, True, 1.0)
-        else:
-            box.scalarProjectionCIC_project(1.0)
-        box.fft(gevpm.PLAN_SOURCE, gevpm.FFT_FORWARD)
-        kb, pk, _, _, occ = box.extractPowerSpectrum(gevpm.SCALARFT, self.numbins or N // 2)
+        out = box.run_to_outputs(sim, st, [req.z_end], self.numbins or N // 2, L)
         phi_sum = box.field_sum(gevpm.PHI)
-        return {"a": st.a, "cycles": st.cycle, "particle_steps": req.npart * st.cycle,
-                "k": kb, "pk": pk, "occupation": occ, "phi_sum": phi_sum, "maxvel": st.maxvel}
+        # st.cycle particle updates ran: the last cycle stops after its field stage
+        return {"a": st.a, "cycles": st.cycle, "particle_steps": req.npart * st.cycle, "z": float(out["z"][0]),
+                "k": out["k"][0], "pk": out["pk"][0], "occupation": out["occupation"][0], "phi_sum": phi_sum,
+                "maxvel": st.maxvel}
 
 
 def default_budget():

@@ -217,7 +217,9 @@ typedef struct {
 typedef struct {
     double a, tau, dtau, dtau_old, maxvel, T00hom, phi_k0;
     int32_t cycle;
-    int32_t reserved0;
+    int32_t stage;  /* 0: at the start of `cycle`; 1: the field stage of `cycle`
+                       has run (gev_run_to_outputs stopped at an output) and its
+                       particle stage comes next */
 } gev_cycle_state;
 
 /* Conformal Hubble rate, background.hpp:137-140 (no ncdm). */
@@ -228,7 +230,9 @@ void gev_rungekutta4bg(double *a, double fourpiG, const gev_cosmology *c, double
 double gev_next_dtau(const gev_sim_params *p, int ngrid, double a);
 
 /* Runs `ncycles` cycles of the main loop (no outputs) and advances *st as
-   main.cpp:825-874 does.  Synchronous at the end (reads T00hom, maxvel).
+   main.cpp:825-874 does; a state left at stage 1 by gev_run_to_outputs
+   completes its cycle's particle stage as the first of them.  Synchronous at
+   the end (reads T00hom, maxvel).
    The cycle applies prepareFTsource inside the first FFT pass and the k-space
    solves inside the z passes, so afterwards source and Sij hold the raw
    deposits, SijFT is scratch and scalarFT is only kept at k = 0; phi, chi,
@@ -314,7 +318,7 @@ int gev_generate_ic_basic(gev_ctx *ctx, const gev_sim_params *p, const gev_ic_pa
 /* hibernate() after a gev_run_cycles call (main.cpp:827-865): writes the
    particles, phi and chi (GR) and B / (a^2 N) (parabolic; a Newtonian run
    first transforms BiFT back), with a, tau, the completed cycle's dtau and
-   its index, to one binary file.  Synchronous. */
+   its index, to one binary file.  Needs a state at stage 0.  Synchronous. */
 int gev_hibernate(gev_ctx *ctx, const gev_sim_params *p, const gev_cycle_state *st, const char *path);
 /* readIC's restart branch (ic_read.hpp:58-330, 671-716): loads the file into
    a context of the same grid, rescales B by a^2 / N^2 and rebuilds BiFT by a
@@ -350,6 +354,38 @@ int gev_write_power_spectrum(const double *kbin, const double *power, const doub
                              double rescalek, double rescalep, const char *filename,
                              const char *description, double a, double z_target);
 
+/* Where each output of gev_run_to_outputs was written. */
+typedef struct {
+    double z;             /* the redshift of the table's header line          */
+    int32_t cycle;        /* the cycle whose field stage it followed          */
+    int32_t interpolated; /* 1: interpolated in redshift with the look-ahead
+                             write of the cycle before                         */
+} gev_pk_output_info;
+
+/* The main loop, main.cpp:372-879, with the delta power-spectrum output
+   (writeSpectra: output.hpp:1796-1853 Newton, CIC mass; 1983-2028 GR, T00
+   with phi) at each of z_pk[0..num_pk) (strictly descending) and
+   EXACT_OUTPUT_REDSHIFTS on (main.cpp:638-677; writePowerSpectrum,
+   tools.hpp:268-346): an output is written after the field stage of the
+   cycle whose a has crossed z_pk[j], interpolated linearly in redshift with
+   the spectrum written one cycle before by the look-ahead, whose values pass
+   through the reference's text table ("%e" powers, "%f" redshift) exactly as
+   the reference reads its own file back.  The loop ends after the field
+   stage of the cycle that writes the last output (main.cpp:685-693), with no
+   particle update, so st->cycle is the number of completed particle updates.
+   Tables [num_pk][numbins] hold what writePowerSpectrum prints: k / boxsize,
+   P, sigma(k), sigma(P), count (rows with count 0 are not printed by the
+   reference).  numbins in [1, 1024]; boxsize in Mpc/h.  Returns GEV_ESTATE
+   if max_cycles particle updates ran before the last output.  Synchronous.
+   On return st->stage is 1; a later call (or gev_run_cycles) continues the
+   loop exactly as if it had not stopped: it first makes the stopped cycle's
+   look-ahead check for its own z_pk[0], then runs that cycle's particle
+   stage. */
+int gev_run_to_outputs(gev_ctx *ctx, const gev_sim_params *p, gev_cycle_state *st, int32_t num_pk,
+                       const double *z_pk, int32_t numbins, double boxsize, int32_t max_cycles,
+                       double *k, double *pk, double *sigma_k, double *sigma_pk, int32_t *occupation,
+                       gev_pk_output_info *info);
+
 /* One line of the background log, main.cpp:520-537: the cycle, tau and a at
    the cycle's start, Hconf(a)/Hconf(1), phi(k=0) after the cycle's Poisson
    solve (scalarFT(k=0).real()) and T00hom (0 for Newtonian runs). */
@@ -361,7 +397,7 @@ typedef struct {
 
 /* gev_run_cycles that also returns one background record per cycle in
    rec[ncycles] (device values copied asynchronously; one synchronisation per
-   1024 cycles). */
+   1024 cycles).  Needs a state at stage 0. */
 int gev_run_cycles_recorded(gev_ctx *ctx, const gev_sim_params *p, gev_cycle_state *st, int ncycles,
                             gev_background_record *rec);
 /* Appends records to a background file in the reference's format

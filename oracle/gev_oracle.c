@@ -1191,19 +1191,19 @@ void orc_box_free(orc_box *b)
     memset(b, 0, sizeof(*b));
 }
 
-/* One cycle of main.cpp:372-879 for one CDM species, no outputs, no ncdm,
-   no radiation.  st->a, tau, dtau, dtau_old, cycle are advanced in place.
-   The particle filing must be current on entry (orc_file_particles) and is
-   current again on exit. */
-void orc_cycle(const orc_sim *sim, orc_box *b, orc_state *st)
+/* The field stage of one cycle of main.cpp:372-879 (projections, both
+   prepareFTsource stencils, the FFTs and k-space solves: main.cpp:372-598)
+   for one CDM species, no ncdm, no radiation.  Leaves phi, chi, Bi, BiFT and
+   st->T00hom, st->phi_k0 as the reference does at the point where it writes
+   its outputs (main.cpp:605-680).  The particle filing must be current. */
+void orc_cycle_fields(const orc_sim *sim, orc_box *b, orc_state *st)
 {
     const int N = sim->N;
     const size_t n3 = (size_t)N * N * N;
     const double dx = 1.0 / (double)N;
     const double fourpiG = sim->fourpiG;
     const orc_cosmo *cs = &sim->cosmo;
-    double a = st->a;
-    double params[2];
+    const double a = st->a;
 
     /* projections, main.cpp:378-450 */
     memset(b->source, 0, sizeof(double) * n3);
@@ -1254,8 +1254,18 @@ void orc_cycle(const orc_sim *sim, orc_box *b, orc_state *st)
     }
     if (sim->gr_flag > 0)
         for (int c = 0; c < 3; c++) orc_fft_c2r(N, b->BiFT + c * nk2, b->Bi + c * n3);
+}
 
-    /* particle update, main.cpp:766-822 */
+/* The particle stage of the cycle (main.cpp:766-822, 867-874): kick, half
+   RK4 step, drift, half RK4 step, re-filing, and the time-step update. */
+void orc_cycle_particles(const orc_sim *sim, orc_box *b, orc_state *st)
+{
+    const int N = sim->N;
+    const double fourpiG = sim->fourpiG;
+    const orc_cosmo *cs = &sim->cosmo;
+    double a = st->a;
+    double params[2];
+
     params[0] = a;
     params[1] = a * a * N;
     if (sim->gr_flag > 0)
@@ -1281,4 +1291,14 @@ void orc_cycle(const orc_sim *sim, orc_box *b, orc_state *st)
     st->a = a;
     st->dtau = orc_next_dtau(sim, a);
     st->cycle++;
+}
+
+/* One cycle of main.cpp:372-879 for one CDM species, no outputs, no ncdm,
+   no radiation.  st->a, tau, dtau, dtau_old, cycle are advanced in place.
+   The particle filing must be current on entry (orc_file_particles) and is
+   current again on exit. */
+void orc_cycle(const orc_sim *sim, orc_box *b, orc_state *st)
+{
+    orc_cycle_fields(sim, b, st);
+    orc_cycle_particles(sim, b, st);
 }

@@ -141,7 +141,24 @@ typedef struct {
 int orc_box_alloc(orc_box *b, int N, int64_t n);
 void orc_box_free(orc_box *b);
 void orc_cycle(const orc_sim *sim, orc_box *b, orc_state *st);
+/* the two halves of orc_cycle: field stage (main.cpp:372-598) and particle
+   stage (main.cpp:766-874) */
+void orc_cycle_fields(const orc_sim *sim, orc_box *b, orc_state *st);
+void orc_cycle_particles(const orc_sim *sim, orc_box *b, orc_state *st);
 double orc_next_dtau(const orc_sim *sim, double a);
+
+/* The main loop (main.cpp:372-879) with delta power-spectrum outputs at
+   z_pk[0..num_pk) (descending; output.hpp:1796-1853 Newton, 1983-2028 GR)
+   and EXACT_OUTPUT_REDSHIFTS (main.cpp:638-677; tools.hpp:268-346): stops
+   after the field stage of the cycle that writes the last output, as the
+   reference's loop breaks (main.cpp:685-693).  Tables [num_pk][numbins]:
+   k / boxsize, P (interpolated in redshift through the reference's text
+   round trip), sigma(k), sigma(P), count, as writePowerSpectrum prints them;
+   z_out[j] the header redshift, cycle_out[j] the cycle.  Returns the number
+   of outputs written, -1 if max_cycles cycles ran first. */
+int orc_run_to_outputs(const orc_sim *sim, orc_box *b, orc_state *st, int num_pk, const double *z_pk,
+                       int numbins, double boxsize, int max_cycles, double *k, double *pk,
+                       double *sigma_k, double *sigma_pk, int *occupation, double *z_out, int *cycle_out);
 
 /* ---- initial conditions (ic_basic.hpp, prng_engine.hpp): gev_oracle_ic.c -- */
 /* counter-based PRNG: block function and the pure stream position -> output */

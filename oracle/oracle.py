@@ -64,6 +64,10 @@ def lib():
         L.orc_box_alloc.argtypes = [_p, _i, _i64]
         L.orc_box_free.argtypes = [_p]
         L.orc_cycle.argtypes = [_p, _p, _p]
+        L.orc_cycle_fields.argtypes = [_p, _p, _p]
+        L.orc_cycle_particles.argtypes = [_p, _p, _p]
+        L.orc_run_to_outputs.restype = _i
+        L.orc_run_to_outputs.argtypes = [_p, _p, _p, _i, _p, _i, _d, _i, _p, _p, _p, _p, _p, _p, _p]
         L.orc_set_threads.argtypes = [_i]
         _u32, _u64 = ctypes.c_uint32, ctypes.c_uint64
         L.orc_prng_block.argtypes = [_u32, _u64, _p]
@@ -494,6 +498,30 @@ class OracleBox:
 
     def cycle(self, sim, state):
         lib().orc_cycle(ctypes.byref(sim), ctypes.byref(self.box), ctypes.byref(state))
+
+    def cycle_fields(self, sim, state):
+        lib().orc_cycle_fields(ctypes.byref(sim), ctypes.byref(self.box), ctypes.byref(state))
+
+    def cycle_particles(self, sim, state):
+        lib().orc_cycle_particles(ctypes.byref(sim), ctypes.byref(self.box), ctypes.byref(state))
+
+    def run_to_outputs(self, sim, state, z_pk, numbins, boxsize, max_cycles=100000):
+        """The main loop with delta power-spectrum outputs at z_pk (descending),
+        EXACT_OUTPUT_REDSHIFTS on; returns a dict of [len(z_pk), numbins]
+        tables (k, pk, sigma_k, sigma_pk, occupation) and per-output z and
+        cycle, or raises if max_cycles ran first."""
+        z = np.ascontiguousarray(z_pk, dtype=np.float64)
+        m = len(z)
+        out = {n: np.zeros((m, numbins)) for n in ("k", "pk", "sigma_k", "sigma_pk")}
+        out["occupation"] = np.zeros((m, numbins), dtype=np.int32)
+        out["z"] = np.zeros(m)
+        out["cycle"] = np.zeros(m, dtype=np.int32)
+        rc = lib().orc_run_to_outputs(ctypes.byref(sim), ctypes.byref(self.box), ctypes.byref(state), m, _ptr(z),
+                                      numbins, boxsize, max_cycles, *[_ptr(out[n]) for n in
+                                      ("k", "pk", "sigma_k", "sigma_pk", "occupation", "z", "cycle")])
+        if rc != m:
+            raise RuntimeError("orc_run_to_outputs: %d of %d outputs written" % (max(rc, 0), m))
+        return out
 
     def close(self):
         if self.box.pos:

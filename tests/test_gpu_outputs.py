@@ -1,0 +1,172 @@
+"""Whole runs to z = 0 on the GPU against the oracle, through the C ABI.
+
+- gev_run_to_outputs (the main loop with the delta power-spectrum output and
+  EXACT_OUTPUT_REDSHIFTS, main.cpp:605-693) against orc_run_to_outputs, GR
+  and Newton, from the same generated initial state.
+- The plain CPU configuration in full: GR, 64^3 grid and particles, box
+  320 Mpc/h, BBKS LCDM ICs from seed 42, z = 100 -> 0.  The GPU and the
+  oracle step in lockstep from the same state; phi, chi, B, positions and
+  momenta are compared after every cycle, cell counts bit for bit, and the
+  spectrum at z = 0 within 1e-6.  A second GPU box re-started from the
+  oracle's state every 10 cycles measures what one step adds.
+
+Tolerances (relative to the field's max norm): 1e-10 for fields and
+positions per cycle, 1e-6 for P(k) at z = 0."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+OUT = os.path.join(ROOT, "gpurun_out")
+C_LIGHT = 2997.92458
+
+
+def rel(a, b):
+    d = np.max(np.abs(np.asarray(a) - np.asarray(b)))
+    s = np.max(np.abs(np.asarray(b)))
+    return d / s if s > 0 else d
+
+
+def bbks_inputs(N, boxsize, gr, seed, Omega_m=0.31, h=0.67):
+    import bbks
+    k, d, t = bbks.bbks_table(np.logspace(-5.0, np.log10(300.0), 400), 100.0, Omega_m, h)
+    tabs = (k * boxsize, d, t * boxsize / h, d, t * boxsize / h)
+    cosmo_o = O.make_cosmo(Omega_m - 0.05, 0.05, h)
+    fpg = 1.5 * boxsize ** 2 / C_LIGHT ** 2
+    sim_o = O.Sim(N, gr, 0, 48.0, 0.04, 100.0, fpg, cosmo_o)
+    ic_o = O.IC(seed, 1, 1, N // 4, 1, 100.0, 2.1e-9, 0.965, 0.05, boxsize)
+    tpl = O.load_template(os.path.join(GOLD, "sc1_crystal.dat"))
+    return tabs, cosmo_o, fpg, sim_o, ic_o, tpl
+
+
+def upload_state(gev, b, ob):
+    """The oracle's whole carried state into a GPU box (particles, phi, chi,
+    B, BiFT and scalarFT)."""
+    b.upload_particles(ob.pos, ob.q, ob.id, ob.box.pmass)
+    for f, name in ((gev.PHI, "phi"), (gev.CHI, "chi"), (gev.BI, "Bi"), (gev.BIFT, "BiFT"),
+                    (gev.SCALARFT, "scalarFT")):
+        b.upload_field(f, getattr(ob, name))
+
+
+def compare(gev, b, ob):
+    """Max relative differences of the carried state; cell-count mismatches."""
+    d = {}
+    for f, name in ((gev.PHI, "phi"), (gev.CHI, "chi"), (gev.BI, "Bi")):
+        d[name] = rel(b.download_field(f), getattr(ob, name))
+    pos, q, ids = b.download_particles()
+    order = np.argsort(ids)
+    dp = np.abs(pos[order] - ob.pos)
+    dp = np.minimum(dp, 1.0 - dp)  # a particle on either side of the periodic edge
+    d["pos"] = float(dp.max())
+    d["q"] = rel(q[order], ob.q)
+    counts, _ = b.download_filing()
+    d["count_mismatch"] = int(np.count_nonzero(counts != ob.count))
+    return d
+
+
+def state_pair(gev, sim_o, cosmo_o, fpg, N):
+    a0 = 1.0 / 101.0
+    dt0 = O.lib().orc_next_dtau(O.ctypes.byref(sim_o), a0)
+    tau0 = O.particle_horizon(a0, fpg, cosmo_o)
+    so = O.State(a0, tau0, dt0, 0.0, 0.0, 0.0, 0.0, 0)
+    sg = gev.CycleState(a0, tau0, dt0, 0.0, 0.0, 0.0, 0.0, 0, 0)
+    sim_g = gev.SimParams(sim_o.gr_flag, 0, 48.0, 0.04, 100.0, fpg,
+                          gev.make_cosmology(cosmo_o.Omega_cdm, cosmo_o.Omega_b, cosmo_o.h))
+    return so, sg, sim_g
+
+
+@pytest.mark.parametrize("gr", [1, 0])
+def test_run_to_outputs_matches_oracle(gev, gr):
+    N, boxsize, numbins = 32, 160.0, 16
+    tabs, cosmo_o, fpg, sim_o, ic_o, tpl = bbks_inputs(N, boxsize, gr, 11)
+    ob = O.OracleBox(N, N ** 3)
+    ob.generate_ic(sim_o, ic_o, tpl, *tabs)
+    ob.file()
+    so, sg, sim_g = state_pair(gev, sim_o, cosmo_o, fpg, N)
+    z_pk = [10.0, 1.0, 0.0]
+    with gev.Box(N, N ** 3) as b:
+        upload_state(gev, b, ob)
+        got = b.run_to_outputs(sim_g, sg, z_pk, numbins, boxsize)
+        want = ob.run_to_outputs(sim_o, so, z_pk, numbins, boxsize)
+        assert np.array_equal(got["cycle"], want["cycle"]) and sg.cycle == so.cycle
+        assert np.array_equal(got["z"], want["z"]) and sg.a == so.a
+        assert got["interpolated"].all()
+        assert np.array_equal(got["occupation"], want["occupation"])
+        live = want["occupation"] > 0
+        err = np.abs(got["pk"] - want["pk"])[live] / np.abs(want["pk"])[live]
+        print("gr=%d P(k) max rel err at z = 10, 1, 0: %.2e (cycles %s)" % (gr, err.max(), got["cycle"].tolist()),
+              flush=True)
+        assert err.max() < 1e-6
+        np.testing.assert_array_equal(got["k"][live], want["k"][live])
+        d = compare(gev, b, ob)
+        assert d["count_mismatch"] == 0 and max(d["phi"], d["chi"], d["Bi"]) < 1e-10, d
+    ob.close()
+
+
+def test_c1_full_run_lockstep_to_z0(gev):
+    N, boxsize, numbins = 64, 320.0, 32
+    tabs, cosmo_o, fpg, sim_o, ic_o, tpl = bbks_inputs(N, boxsize, 1, 42)
+    ob = O.OracleBox(N, N ** 3)
+    ob.generate_ic(sim_o, ic_o, tpl, *tabs)
+    so, sg, sim_g = state_pair(gev, sim_o, cosmo_o, fpg, N)
+    growth, per_step = [], []
+    with gev.Box(N, N ** 3) as b, gev.Box(N, N ** 3) as b1:
+        # the GPU's own generator against the oracle's
+        b.generateIC_basic(sim_g, gev.ICParams(42, 1, 1, N // 4, 100.0, 2.1e-9, 0.965, 0.05, boxsize), tpl, *tabs)
+        ic_err = {name: rel(b.download_field(f), getattr(ob, name))
+                  for f, name in ((gev.PHI, "phi"), (gev.CHI, "chi"), (gev.BI, "Bi"))}
+        assert max(ic_err.values()) < 1e-10, ic_err
+        ob.file()
+        upload_state(gev, b, ob)
+        while True:
+            # stop where the next cycle's look-ahead writes the z = 0 spectrum
+            tmp = gev.rungekutta4bg(gev.rungekutta4bg(sg.a, fpg, sim_g.cosmo, 0.5 * sg.dtau), fpg, sim_g.cosmo,
+                                    0.5 * sg.dtau)
+            if 1.0 / tmp < 1.0:
+                break
+            if sg.cycle % 10 == 0:  # one step from the oracle's state
+                upload_state(gev, b1, ob)
+                s1 = gev.CycleState(sg.a, sg.tau, sg.dtau, sg.dtau_old, 0.0, 0.0, 0.0, sg.cycle, 0)
+                b1.run_cycles(sim_g, s1, 1)
+            ob.cycle(sim_o, so)
+            b.run_cycles(sim_g, sg, 1)
+            assert (sg.a, sg.tau, sg.dtau, sg.cycle) == (so.a, so.tau, so.dtau, so.cycle)
+            d = compare(gev, b, ob)
+            d["cycle"] = so.cycle
+            growth.append(d)
+            if (so.cycle - 1) % 10 == 0:
+                d1 = compare(gev, b1, ob)
+                d1["cycle"] = so.cycle
+                per_step.append(d1)
+                assert d1["count_mismatch"] == 0, d1
+                assert max(d1["phi"], d1["chi"], d1["Bi"]) < 1e-10 and d1["q"] < 1e-10 and d1["pos"] < 1e-12, d1
+        got = b.run_to_outputs(sim_g, sg, [0.0], numbins, boxsize)
+        want = ob.run_to_outputs(sim_o, so, [0.0], numbins, boxsize)
+        growth.append(dict(compare(gev, b, ob), cycle=so.cycle))
+    ob.close()
+    live = want["occupation"][0] > 0
+    pk_err = float(np.max(np.abs(got["pk"][0] - want["pk"][0])[live] / want["pk"][0][live]))
+    worst = {k: max(g[k] for g in growth) for k in ("phi", "chi", "Bi", "pos", "q", "count_mismatch")}
+    report = {"config": "GR 64^3, box 320 Mpc/h, BBKS seed 42, z = 100 -> 0", "cycles": int(sg.cycle),
+              "ic_generator_err": ic_err, "pk_z0_max_rel_err": pk_err, "lockstep_worst": worst,
+              "lockstep": growth, "one_step": per_step}
+    os.makedirs(OUT, exist_ok=True)
+    with open(os.path.join(OUT, "c1_full_run.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    print("C1 to z = 0: %d cycles, P(k) err %.2e, lockstep worst %s" % (sg.cycle, pk_err, worst), flush=True)
+    print("one step:", [(p["cycle"], "%.1e" % max(p["phi"], p["chi"], p["Bi"], p["q"])) for p in per_step],
+          flush=True)
+    assert got["cycle"][0] == want["cycle"][0] and got["z"][0] == want["z"][0] == 0.0
+    assert np.array_equal(got["occupation"], want["occupation"])
+    assert pk_err < 1e-6
+    # the whole run from one shared initial state: differences stay at the
+    # per-step level through the nonlinear growth to z = 0
+    assert worst["count_mismatch"] == 0
+    assert max(worst["phi"], worst["chi"], worst["Bi"], worst["q"]) < 1e-10 and worst["pos"] < 1e-12, worst

@@ -42,7 +42,7 @@ def test_results_alone_equal_results_in_the_mix_and_on_reused_boxes():
             (o,) = s.serve([r])
         assert o.status == "ok", o.error
         alone[r.rid] = o.result
-        assert o.result["a"] >= 1.0 and o.result["cycles"] > 50
+        assert o.result["a"] >= 1.0 and o.result["cycles"] > 50 and o.result["z"] == 0.0
         assert np.all(np.isfinite(o.result["pk"])) and np.max(o.result["pk"]) > 0
     print("alone:", {k: (v["cycles"], v["phi_sum"]) for k, v in alone.items()}, flush=True)
     with sv.Server(be, sv.default_budget(), slots=4, reuse=True) as s:
@@ -60,30 +60,36 @@ def test_results_alone_equal_results_in_the_mix_and_on_reused_boxes():
         _same(o.result, alone[o.rid])
 
 
-def test_spectrum_of_a_served_run_equals_the_direct_calls():
-    """The serving backend runs exactly the C-ABI sequence a driver would:
-    IC generation, the cycles to z = 0, the delta power spectrum."""
+def test_served_run_matches_the_oracle():
+    """A served request (IC generation, the main loop to z = 0 with the
+    delta spectrum written as EXACT_OUTPUT_REDSHIFTS writes it) against the
+    oracle's generator and output loop on the same inputs: same cycles,
+    occupations exact, P(k) at z = 0 within 1e-6."""
     import gevpm as gev
     import serving as sv
-    tpl = gev.load_homogeneous_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
-    r = sv.Request(0, 32, gr=True, Omega_m=0.3, A_s=2.1e-9, seed=5)
+    import bbks
+    import oracle as O
+    N = 32
+    tpl = O.load_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
+    r = sv.Request(0, N, gr=True, Omega_m=0.3, A_s=2.1e-9, seed=5)
     with sv.Server(sv.GevBackend(template=tpl), sv.default_budget(), slots=1, reuse=False) as s:
         (o,) = s.serve([r])
     assert o.status == "ok", o.error
-    import bbks
-    sim, st = sv.GevBackend.setup(r)
     L, h = r.box_mpc, r.h
     k, d, t = bbks.bbks_table(np.logspace(-5.0, np.log10(300.0), 400), 100.0, r.Omega_m, h)
-    with gev.Box(32, 32 ** 3) as b:
-        ic = gev.ICParams(5, 1, 1, 8, 100.0, 2.1e-9, 0.965, 0.05, L)
-        st.maxvel, _ = b.generateIC_basic(sim, ic, tpl, k * L, d, t * L / h, d, t * L / h)
-        while 1.0 / st.a - 1.0 > 1e-12:
-            b.run_cycles(sim, st, 1)
-        b.projection_init(gev.SOURCE)
-        b.projection_T00_project(st.a, True, 1.0)
-        b.fft(gev.PLAN_SOURCE, gev.FFT_FORWARD)
-        p = b.extractPowerSpectrum(gev.SCALARFT, 16)
-        phi_sum = b.field_sum(gev.PHI)
-    assert o.result["cycles"] == st.cycle and o.result["a"] == st.a
-    np.testing.assert_array_equal(o.result["pk"], p[1])
-    assert o.result["phi_sum"] == phi_sum
+    cosmo = O.make_cosmo(r.Omega_m - r.Omega_b, r.Omega_b, h)
+    sim = O.Sim(N, 1, 0, 48.0, 0.04, 100.0, 1.5 * L * L / sv.C_LIGHT ** 2, cosmo)
+    ob = O.OracleBox(N, N ** 3)
+    ob.generate_ic(sim, O.IC(5, 1, 1, N // 4, 1, 100.0, 2.1e-9, 0.965, 0.05, L), tpl,
+                   k * L, d, t * L / h, d, t * L / h)
+    a0 = 1.0 / 101.0
+    st = O.State(a0, 0.0, O.lib().orc_next_dtau(O.ctypes.byref(sim), a0), 0.0, 0.0, 0.0, 0.0, 0)
+    want = ob.run_to_outputs(sim, st, [0.0], N // 2, L)
+    ob.close()
+    res = o.result
+    assert res["cycles"] == st.cycle and res["a"] == st.a and res["z"] == want["z"][0] == 0.0
+    np.testing.assert_array_equal(res["occupation"], want["occupation"][0])
+    live = res["occupation"] > 0
+    err = np.max(np.abs(res["pk"] - want["pk"][0])[live] / want["pk"][0][live])
+    print("served 32^3 GR run: %d cycles, P(k) at z = 0 within %.2e of the oracle" % (st.cycle, err), flush=True)
+    assert err < 1e-6

@@ -171,26 +171,6 @@ def test_poisson_trace():
     assert a[0].box_mpc == 5.0 * a[0].ngrid
 
 
-def test_cycles_to_z0_follow_the_cycle_recurrence():
-    """The host count of cycles to z = 0 for the 512^3, 2048 Mpc/h run is
-    the 117 cycles the device run takes, and it stops at the first cycle
-    that reaches z <= 0."""
-    gev = pytest.importorskip("gevpm")
-    if not os.path.exists(gev.LIB_PATH):
-        pytest.skip("library not built")
-    r = sv.Request(0, 512, boxsize=2048.0)
-    sim, st = sv.GevBackend.setup(r)
-    n = sv.GevBackend.cycles_to(sim, st, 512, 0.0)
-    assert n == 117
-    a, dtau = st.a, st.dtau
-    for i in range(n):
-        assert 1.0 / a - 1.0 > 0.0
-        a = gev.rungekutta4bg(a, sim.fourpiG, sim.cosmo, 0.5 * dtau)
-        a = gev.rungekutta4bg(a, sim.fourpiG, sim.cosmo, 0.5 * dtau)
-        dtau = gev.next_dtau(sim, 512, a)
-    assert 1.0 / a - 1.0 <= 0.0
-
-
 def test_default_template_is_the_sc1_crystal():
     gev = pytest.importorskip("gevpm")
     if not os.path.exists(gev.LIB_PATH):
@@ -214,27 +194,13 @@ class RecordingBox:
         self.calls.append(("ic", sim.gr_flag, ic.seed, ic.numtile, ic.boxsize, ic.A_s, len(tpl), float(k[0])))
         return 1e-3, 0.1
 
-    def run_cycles(self, sim, st, n):
-        self.calls.append(("cycles", n))
-        st.cycle += n
-        st.a = 1.0
-
-    def projection_init(self, f):
-        self.calls.append(("init", f))
-
-    def projection_T00_project(self, a, use_phi, coeff):
-        self.calls.append(("T00", a, use_phi))
-
-    def scalarProjectionCIC_project(self, coeff):
-        self.calls.append(("CIC",))
-
-    def fft(self, plan, direction):
-        self.calls.append(("fft", plan, direction))
-
-    def extractPowerSpectrum(self, f, numbins):
-        self.calls.append(("pk", f, numbins))
-        z = np.zeros(numbins)
-        return z, z + 1.0, z, z, np.ones(numbins, dtype=np.int32)
+    def run_to_outputs(self, sim, st, z_pk, numbins, boxsize):
+        self.calls.append(("outputs", list(z_pk), numbins, boxsize))
+        st.cycle += 117
+        st.a = 1.001
+        z = np.zeros((1, numbins))
+        return {"k": z, "pk": z + 1.0, "sigma_k": z, "sigma_pk": z, "occupation": np.ones((1, numbins), dtype=np.int32),
+                "z": np.zeros(1), "cycle": np.array([117]), "interpolated": np.array([True])}
 
     def field_sum(self, f):
         return 0.5
@@ -243,27 +209,36 @@ class RecordingBox:
 @pytest.mark.parametrize("gr", [True, False])
 def test_backend_runs_the_reference_sequence(gr):
     """IC generation with the request's seed, tiling N/4 and a 5 Mpc/h-per-cell
-    box; the host-counted cycles to z = 0 in one call; then the delta
-    spectrum (T00 with phi for GR, the CIC mass for Newtonian runs) in N/2
-    bins."""
+    box, then the main loop with the delta spectrum at z = 0 in N/2 bins
+    (gev_run_to_outputs); the particle steps are the completed updates."""
     gev = pytest.importorskip("gevpm")
     if not os.path.exists(gev.LIB_PATH):
         pytest.skip("library not built")
     r = sv.Request(3, 32, gr=gr, Omega_m=0.3, A_s=2.0e-9, seed=11)
     sim, st = sv.GevBackend.setup(r)
-    n = sv.GevBackend.cycles_to(sim, st, 32, 0.0)
     box = RecordingBox()
     res = sv.GevBackend().run(box, r)
     ic = box.calls[0]
     assert ic[:6] == ("ic", 1 if gr else 0, 11, 8, 160.0, 2.0e-9) and ic[6] == 64
     assert abs(ic[7] - 1e-5 * 160.0) < 1e-18  # k in units of 1/box (loadTransferFunctions)
-    assert box.calls[1] == ("cycles", n)
-    proj = ("T00", 1.0, True) if gr else ("CIC",)
-    assert box.calls[2:] == [("init", gev.SOURCE), proj, ("fft", gev.PLAN_SOURCE, gev.FFT_FORWARD),
-                             ("pk", gev.SCALARFT, 16)]
-    assert res["cycles"] == n and res["particle_steps"] == 32 ** 3 * n
+    assert box.calls[1:] == [("outputs", [0.0], 16, 160.0)]
+    assert res["cycles"] == 117 and res["particle_steps"] == 32 ** 3 * 117 and res["z"] == 0.0
     assert sim.fourpiG == 1.5 * 160.0 ** 2 / sv.C_LIGHT ** 2
     assert abs(sim.cosmo.Omega_cdm - 0.25) < 1e-15 and sim.cosmo.Omega_b == 0.05
+
+
+def test_waiting_for_running_boxes_keeps_the_pool():
+    """A head request that cannot fit beside the running boxes, even with
+    the pool emptied, waits without evicting pooled boxes."""
+    be = FakeBackend()
+    s = sv.Server(be, 3 * 16 ** 3, slots=4)
+    big, small = sv.Request(0, 16), sv.Request(1, 12)
+    got = s._take(big, 2 * 16 ** 3)           # a running box
+    assert got is not None
+    s._give_back(small, s._take(small, 12 ** 3)[0], 12 ** 3, True)  # a pooled box
+    assert s._take(sv.Request(2, 16), 2 * 16 ** 3) is None
+    assert len(s._pool) == 1 and s.evicted == 0
+    s.close()
 
 
 def test_a_request_that_cannot_fit_beside_held_memory_is_rejected_not_waited_for():
