@@ -116,13 +116,15 @@ static int64_t footprint(int N, int64_t cap, bool fp32) {
     b += (s.n3 / 256 + 1 + sort_big_cap(s.cap) + 2) * 4;  // the ID sort's lists and counts
     b += s.nred * 8 + 8;
     b += N * (16 + 8 + 16);
+    b += spectrum_scratch_doubles(N, spectrum_bins_cap(N)) * 8;
     return b;
 }
 
 static void free_all(gev_ctx *c) {
     void *ptrs[] = {c->phi, c->chi, c->source, c->Sij, c->Bi, c->scalarFT, c->SijFT, c->BiFT,
                     c->fft_tmp, c->twiddle, c->twiddle_f, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
-                    c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u, c->sort_segs, c->sort_big, c->sort_n};
+                    c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u, c->sort_segs, c->sort_big, c->sort_n,
+                    c->pk_scratch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
@@ -222,6 +224,8 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     A((void **)&c->sort_n, 8);
     A((void **)&c->red_d, s.nred * 8);
     A((void **)&c->red_u, 64);
+    c->pk_bins = spectrum_bins_cap(c->N);
+    A((void **)&c->pk_scratch, spectrum_scratch_doubles(c->N, c->pk_bins) * 8);
     if (ok && hipHostMalloc((void **)&c->host_pinned, 64) != hipSuccess) ok = false;
     if (ok && hipHostMalloc((void **)&c->rec_pinned, sizeof(double) * 2 * REC_SLOTS) != hipSuccess) ok = false;
     if (!ok) {

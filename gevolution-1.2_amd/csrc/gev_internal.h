@@ -78,6 +78,11 @@ struct gev_ctx {
     uint32_t *sort_segs = nullptr, *sort_big = nullptr, *sort_n = nullptr;
     int64_t scan_nblk = 0;
 
+    // power-spectrum scratch (tables, workgroup histograms, bin totals) for
+    // up to pk_bins bins
+    double *pk_scratch = nullptr;
+    int pk_bins = 0;
+
     // reductions
     double *red_d = nullptr;              // device double scratch (sums)
     unsigned long long *red_u = nullptr;  // device max-as-bits scratch
@@ -177,6 +182,10 @@ int launch_ic_displace(gev_ctx *c, const double *field, double coeff);  // max -
 int launch_ic_q(gev_ctx *c, const double *field, double coeff);         // max |q|^2 -> red_u
 // binned (cross) spectrum of k fields (spectrum.hip); mode 0: components
 // (comp1, comp2), 1: symmetric tensor, 2: sum over components
+// doubles of power-spectrum scratch for numbins bins at grid N; the bins a
+// context's own scratch holds
+int64_t spectrum_scratch_doubles(int N, int numbins);
+inline int spectrum_bins_cap(int N) { return N / 2 > 64 ? N / 2 : 64; }
 int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mode, int comp1, int comp2,
              int numbins, int deconvolve, int ktype, double *kbin, double *power, double *kscatter,
              double *pscatter, int32_t *occupation);

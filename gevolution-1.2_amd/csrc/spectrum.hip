@@ -11,6 +11,7 @@
 // workgroup order.  The per-bin means and scatters of the reference's root
 // process are then formed on the host.
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "gev_internal.h"
@@ -137,12 +138,8 @@ __global__ __launch_bounds__(TPB) void k_spectrum_reduce(const double *__restric
 
 namespace gev {
 
-int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mode, int comp1, int comp2,
-             int numbins, int deconvolve, int ktype, double *kbin, double *power, double *kscatter,
-             double *pscatter, int32_t *occupation) {
-    const int N = c->N;
-    // typek2 and sinc of tools.hpp:66-104, on the host (same expressions)
-    std::vector<double> tk(N), sn(N);
+// tools.hpp:66-104: typek2 and sinc tables, on the host (same expressions)
+static void spectrum_tables(int N, int deconvolve, int ktype, double *tk, double *sn) {
     int i;
     if (ktype == 0) {
         for (i = 0; i < N; i++) {
@@ -162,25 +159,38 @@ int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mo
     sn[0] = 1.;
     for (i = 1; i <= N / 2; i++) sn[i] = deconvolve ? sin(M_PI * (float)i / (float)N) * (float)N / (M_PI * (float)i) : 1.;
     for (; i < N; i++) sn[i] = sn[N - i];
+}
 
+// workgroups of the row pass: about 1024, each a run of whole rows
+static int spectrum_blocks(int N, int64_t *rows_per_block) {
     const int64_t rows = (int64_t)N * N;
-    const int64_t rpb = (rows + 1023) / 1024;  // about 1024 workgroups
-    const int nblk = (int)((rows + rpb - 1) / rpb);
-    void *dtk = nullptr, *dsn = nullptr, *dpart = nullptr, *dtot = nullptr;
-    auto release = [&]() {
-        for (void *p : {dtk, dsn, dpart, dtot})
-            if (p) (void)hipFree(p);
-    };
-    int rc = check_hip(c, hipMalloc(&dtk, N * 8), "spectrum tables");
-    if (!rc) rc = check_hip(c, hipMalloc(&dsn, N * 8), "spectrum tables");
-    if (!rc) rc = check_hip(c, hipMalloc(&dpart, (size_t)nblk * numbins * NQ * 8), "spectrum partials");
-    if (!rc) rc = check_hip(c, hipMalloc(&dtot, (size_t)numbins * NQ * 8), "spectrum totals");
-    if (!rc) rc = check_hip(c, hipMemcpy(dtk, tk.data(), N * 8, hipMemcpyHostToDevice), "spectrum tables");
-    if (!rc) rc = check_hip(c, hipMemcpy(dsn, sn.data(), N * 8, hipMemcpyHostToDevice), "spectrum tables");
-    if (rc) {
-        release();
-        return rc;
-    }
+    const int64_t rpb = (rows + 1023) / 1024;
+    *rows_per_block = rpb;
+    return (int)((rows + rpb - 1) / rpb);
+}
+
+int64_t spectrum_scratch_doubles(int N, int numbins) {
+    int64_t rpb;
+    const int nblk = spectrum_blocks(N, &rpb);
+    return 2 * (int64_t)N + ((int64_t)nblk + 1) * numbins * NQ;
+}
+
+// The spectrum with its device scratch given: the two tables (2N doubles),
+// the workgroup histograms and the bin totals, spectrum_scratch_doubles(N,
+// numbins) doubles in all.  tk, sn, tot: host buffers of N, N and numbins * NQ
+// doubles.  Copies are ordered on the context stream; synchronous.
+static int spectrum_core(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mode, int comp1,
+                         int comp2, int numbins, int deconvolve, int ktype, double *scratch, double *tk, double *sn,
+                         double *tot) {
+    const int N = c->N;
+    spectrum_tables(N, deconvolve, ktype, tk, sn);
+    int64_t rpb;
+    const int nblk = spectrum_blocks(N, &rpb);
+    double *dtk = scratch, *dsn = scratch + N, *dpart = scratch + 2 * N;
+    double *dtot = dpart + (int64_t)nblk * numbins * NQ;
+    int rc = check_hip(c, hipMemcpyAsync(dtk, tk, N * 8, hipMemcpyHostToDevice, c->stream), "spectrum tables");
+    if (!rc) rc = check_hip(c, hipMemcpyAsync(dsn, sn, N * 8, hipMemcpyHostToDevice, c->stream), "spectrum tables");
+    if (rc) return rc;
     PkArgs A;
     A.f1 = f1;
     A.f2 = f2;
@@ -189,14 +199,14 @@ int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mo
     A.mode = mode;
     A.comp1 = comp1;
     A.comp2 = comp2;
-    A.typek2 = (const double *)dtk;
-    A.sinc = (const double *)dsn;
+    A.typek2 = dtk;
+    A.sinc = dsn;
     A.k2max = 3. * tk[N / 2];
     A.N = N;
     A.P = c->P;
     A.numbins = numbins;
     A.rows_per_block = rpb;
-    A.partial = (double *)dpart;
+    A.partial = dpart;
     {
         ScopedTimer t(c, "power_spectrum");
         const size_t shm = (size_t)numbins * NQ * 8;
@@ -228,15 +238,18 @@ int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mo
             hipLaunchKernelGGL((k_spectrum_rows<513, double2>), dim3(nblk), dim3(TPB), shm, c->stream, A);
         }
         hipLaunchKernelGGL(k_spectrum_reduce, dim3((numbins * NQ + TPB - 1) / TPB), dim3(TPB), 0, c->stream,
-                           (const double *)dpart, nblk, numbins, (double *)dtot);
+                           (const double *)dpart, nblk, numbins, dtot);
     }
     rc = check_hip(c, hipGetLastError(), "spectrum launch");
-    std::vector<double> tot((size_t)numbins * NQ);
+    if (!rc) rc = check_hip(c, hipMemcpyAsync(tot, dtot, (size_t)numbins * NQ * 8, hipMemcpyDeviceToHost, c->stream),
+                            "spectrum readback");
     if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "spectrum sync");
-    if (!rc) rc = check_hip(c, hipMemcpy(tot.data(), dtot, tot.size() * 8, hipMemcpyDeviceToHost), "spectrum readback");
-    release();
-    if (rc) return rc;
-    // the root process's moments (tools.hpp:180-194)
+    return rc;
+}
+
+// the root process's moments (tools.hpp:180-194) from the bin totals
+static void spectrum_moments(const double *tot, int numbins, double *kbin, double *power, double *kscatter,
+                             double *pscatter, int32_t *occupation) {
     for (int b = 0; b < numbins; b++) {
         const double *t = &tot[(size_t)b * NQ];
         kbin[b] = t[0];
@@ -253,10 +266,69 @@ int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mo
             if (!std::isfinite(pscatter[b])) pscatter[b] = 0.;
         }
     }
+}
+
+// The spectrum on the context's own scratch (allocated with the context for
+// up to spectrum_bins_cap(N) bins; more bins take a scratch allocated for the
+// call).
+int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mode, int comp1, int comp2,
+             int numbins, int deconvolve, int ktype, double *kbin, double *power, double *kscatter,
+             double *pscatter, int32_t *occupation) {
+    const int N = c->N;
+    std::vector<double> tk(N), sn(N), tot((size_t)numbins * NQ);
+    double *scratch = c->pk_scratch;
+    void *temp = nullptr;
+    if (numbins > c->pk_bins) {
+        int rc = check_hip(c, hipMalloc(&temp, spectrum_scratch_doubles(N, numbins) * 8), "spectrum scratch");
+        if (rc) return rc;
+        scratch = (double *)temp;
+    }
+    int rc = spectrum_core(c, f1, f2, ncomp, mode, comp1, comp2, numbins, deconvolve, ktype, scratch, tk.data(),
+                           sn.data(), tot.data());
+    if (temp) (void)hipFree(temp);
+    if (rc) return rc;
+    spectrum_moments(tot.data(), numbins, kbin, power, kscatter, pscatter, occupation);
     return 0;
 }
 
 }  // namespace gev
+
+// ---------------------------------------------------------------- static scratch
+
+// Device scratch of gev_extract_power_spectrum_noalloc: the tables, the
+// workgroup histograms and the bin totals for up to GEV_PK_NOALLOC_BINS bins
+// at N <= 1024 (about 10.5 MB per device, held until the process ends).
+// Every call on a device shares that device's copy.
+#define GEV_PK_NOALLOC_BINS 256
+#define GEV_PK_NOALLOC_N 1024
+#define GEV_PK_NOALLOC_DOUBLES (2 * GEV_PK_NOALLOC_N + (1024 + 1) * GEV_PK_NOALLOC_BINS * 5)
+static __device__ __align__(256) double scratch[GEV_PK_NOALLOC_DOUBLES];
+
+namespace {
+
+// the device address of `scratch` per device: looked up once per device, with
+// that device current, on the first call there
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+double *scratch_ptr[MAX_DEVICES] = {};
+
+int static_scratch(gev_ctx *c, double **out) {
+    int dev = -1;
+    int rc = gev::check_hip(c, hipGetDevice(&dev), "current device");
+    if (rc) return rc;
+    if (dev < 0 || dev >= MAX_DEVICES) return gev::fail(c, GEV_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        rc = gev::check_hip(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "static scratch address");
+        if (rc) return rc;
+        scratch_ptr[dev] = (double *)p;
+    }
+    *out = scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------- C ABI
 
@@ -294,6 +366,29 @@ GEV_EXPORT int gev_extract_power_spectrum(gev_ctx *c, gev_field f, int32_t numbi
                                           double *pscatter, int32_t *occupation) {
     return gev_extract_cross_spectrum(c, f, f, numbins, deconvolve, ktype, -1, -1, kbin, power, kscatter, pscatter,
                                       occupation);
+}
+
+GEV_EXPORT int gev_extract_power_spectrum_noalloc(gev_ctx *c, gev_field f, int32_t numbins, int32_t deconvolve,
+                                                  int32_t ktype, double *kbin, double *power, double *kscatter,
+                                                  double *pscatter, int32_t *occupation) {
+    if (!c) return GEV_EINVAL;
+    const double2 *a;
+    int na;
+    int rc = kfield(c, f, &a, &na);
+    if (rc) return rc;
+    if (numbins < 1 || numbins > GEV_PK_NOALLOC_BINS)
+        return gev::fail(c, GEV_EINVAL, "numbins must be in [1, 256] for the allocation-free spectrum");
+    if (c->N > GEV_PK_NOALLOC_N) return gev::fail(c, GEV_EINVAL, "the allocation-free spectrum takes N <= 1024");
+    if (!kbin || !power || !kscatter || !pscatter || !occupation) return gev::fail(c, GEV_EINVAL, "null output");
+    double *dev = nullptr;
+    rc = static_scratch(c, &dev);
+    if (rc) return rc;
+    // host buffers on the stack: tables and bin totals
+    double tk[GEV_PK_NOALLOC_N], sn[GEV_PK_NOALLOC_N], tot[GEV_PK_NOALLOC_BINS * 5];
+    rc = gev::spectrum_core(c, a, a, na, na == 6 ? 1 : 2, -1, -1, numbins, deconvolve, ktype, dev, tk, sn, tot);
+    if (rc) return rc;
+    gev::spectrum_moments(tot, numbins, kbin, power, kscatter, pscatter, occupation);
+    return 0;
 }
 
 GEV_EXPORT int gev_write_power_spectrum(const double *kbin, const double *power, const double *kscatter,

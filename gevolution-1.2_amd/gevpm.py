@@ -47,7 +47,7 @@ EXPORTED = [
     "gev_initialize_particle_positions", "gev_displace_particles_ic", "gev_initialize_q_ic",
     "gev_generate_ic_basic", "gev_extract_cross_spectrum", "gev_extract_power_spectrum",
     "gev_write_power_spectrum", "gev_hibernate", "gev_restart", "gev_run_cycles_recorded",
-    "gev_write_background", "gev_copy_state", "gev_run_to_outputs",
+    "gev_write_background", "gev_copy_state", "gev_run_to_outputs", "gev_extract_power_spectrum_noalloc",
 ]
 
 
@@ -167,6 +167,8 @@ def lib():
                                                  _p, _p, _p, _p, _p]
         L.gev_write_power_spectrum.argtypes = [_p, _p, _p, _p, _p, ctypes.c_int32, _d, _d, ctypes.c_char_p,
                                                ctypes.c_char_p, _d, _d]
+        L.gev_extract_power_spectrum_noalloc.argtypes = [_p, _i, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                         _p, _p, _p, _p, _p]
         L.gev_run_to_outputs.argtypes = [_p, _p, _p, ctypes.c_int32, _p, ctypes.c_int32, _d, ctypes.c_int32,
                                          _p, _p, _p, _p, _p, _p]
         _lib = L
@@ -446,6 +448,14 @@ class Box:
         out = [np.zeros(numbins) for _ in range(4)] + [np.zeros(numbins, dtype=np.int32)]
         self._chk(lib().gev_extract_power_spectrum(self._c, f, numbins, int(deconvolve), ktype,
                                                    *[_ptr(o) for o in out]), "power spectrum")
+        return tuple(out)
+
+    def extractPowerSpectrum_noalloc(self, f, numbins, deconvolve=True, ktype=1):
+        """extractPowerSpectrum through the entry point that allocates
+        nothing (static device scratch; numbins <= 256, N <= 1024)."""
+        out = [np.zeros(numbins) for _ in range(4)] + [np.zeros(numbins, dtype=np.int32)]
+        self._chk(lib().gev_extract_power_spectrum_noalloc(self._c, f, numbins, int(deconvolve), ktype,
+                                                           *[_ptr(o) for o in out]), "power spectrum (noalloc)")
         return tuple(out)
 
     def run_cycles(self, sim, state, ncycles):

@@ -345,6 +345,18 @@ int gev_extract_cross_spectrum(gev_ctx *ctx, gev_field f1, gev_field f2, int32_t
 int gev_extract_power_spectrum(gev_ctx *ctx, gev_field f, int32_t numbins, int32_t deconvolve,
                                int32_t ktype, double *kbin, double *power, double *kscatter,
                                double *pscatter, int32_t *occupation);
+/* gev_extract_power_spectrum that allocates nothing: no counterpart in the
+   reference; for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid it).  Its device
+   scratch is a static array of the library (about 10.5 MB per device, held
+   until the process ends) shared by every call on that device, so calls that
+   overlap on one device race; every call overwrites what it uses.  Limits:
+   numbins <= 256 and N <= 1024 (GEV_EINVAL beyond).  Results equal
+   gev_extract_power_spectrum's bit for bit.  Synchronous. */
+int gev_extract_power_spectrum_noalloc(gev_ctx *ctx, gev_field f, int32_t numbins, int32_t deconvolve,
+                                       int32_t ktype, double *kbin, double *power, double *kscatter,
+                                       double *pscatter, int32_t *occupation);
+
 /* writePowerSpectrum (tools.hpp:268-346) with EXACT_OUTPUT_REDSHIFTS: when
    1/a < z_target + 1 and `filename` already holds the spectrum written before
    z_target was crossed, the power is interpolated linearly in redshift to

@@ -104,7 +104,8 @@ def test_run_to_outputs_matches_oracle(gev, gr):
         print("gr=%d P(k) max rel err at z = 10, 1, 0: %.2e (cycles %s)" % (gr, err.max(), got["cycle"].tolist()),
               flush=True)
         assert err.max() < 1e-6
-        np.testing.assert_array_equal(got["k"][live], want["k"][live])
+        # bin means: the device sums a bin's modes in another order (1e-12, as the spectrum tests)
+        np.testing.assert_allclose(got["k"][live], want["k"][live], rtol=1e-12, atol=0)
         d = compare(gev, b, ob)
         assert d["count_mismatch"] == 0 and max(d["phi"], d["chi"], d["Bi"]) < 1e-10, d
     ob.close()
