@@ -104,52 +104,30 @@ static Sizes sizes_of(int N, int64_t cap) {
     return s;
 }
 
-static int64_t footprint(int N, int64_t cap, bool fp32) {
-    Sizes s = sizes_of(N, cap);
-    const int64_t rb = fp32 ? 4 : 8, cb = 2 * rb;
-    int64_t b = 0;
-    b += 12 * s.n3 * rb;            // phi, chi, source, Sij(6), Bi(3)
-    b += 11 * s.nk * cb;            // scalarFT, SijFT(6), BiFT(3), fft scratch
-    b += fp32 ? N * 8 : 0;          // float twiddles
-    b += s.cap * (2 * (6 * 8 + 8) + 2 * 4);  // pos, q, id (two sets), cell, perm
-    b += s.n3 * 4 + (s.n3 + 1) * 4 + s.nblk_scan * 4;
-    b += (s.n3 / 256 + 1 + sort_big_cap(s.cap) + 2) * 4;  // the ID sort's lists and counts
-    b += s.nred * 8 + 8;
-    b += N * (16 + 8 + 16);
-    b += spectrum_scratch_doubles(N, spectrum_bins_cap(N)) * 8;
-    return b;
-}
+int64_t slab_bytes(int N, int64_t cap, bool fp32);
+
+static int64_t footprint(int N, int64_t cap, bool fp32) { return slab_bytes(N, cap, fp32); }
 
 static void free_all(gev_ctx *c) {
-    void *ptrs[] = {c->phi, c->chi, c->source, c->Sij, c->Bi, c->scalarFT, c->SijFT, c->BiFT,
-                    c->fft_tmp, c->twiddle, c->twiddle_f, c->gridk2, c->kshift, c->pos, c->q, c->id, c->pos2, c->q2, c->id2, c->cell,
-                    c->perm, c->count, c->start, c->scan_part, c->red_d, c->red_u, c->sort_segs, c->sort_big, c->sort_n,
-                    c->pk_scratch};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    if (c->view) return;  // a view owns nothing
+    if (c->slab) (void)hipFree(c->slab);
+    c->slab = nullptr;
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
     if (c->rec_pinned) (void)hipHostFree(c->rec_pinned);
+    c->host_pinned = c->rec_pinned = nullptr;
 }
 
 // ---------------------------------------------------------------- cycle
 
 static double hconf(double a, double fourpiG, const gev_cosmology *c) {
     // background.hpp:137-140 with bg_ncdm = 0
-    return sqrt((2. * fourpiG / 3.) *
-                (((c->Omega_cdm + c->Omega_b + 0.0) / a) + (c->Omega_Lambda * a * a) +
-                 (c->Omega_rad / a / a) +
-                 (c->Omega_fld * exp(3. * c->wa_fld * (a - 1.)) /
-                  pow(a, 1. + 3. * (c->w0_fld + c->wa_fld)))));
+    return hconf_of(a, fourpiG, c->Omega_cdm, c->Omega_b, c->Omega_Lambda, c->Omega_rad, c->Omega_fld, c->w0_fld,
+                    c->wa_fld);
 }
 
 static void rk4bg(double &a, double fourpiG, const gev_cosmology *c, double dtau) {
     // background.hpp:167-177
-    double k1a, k2a, k3a, k4a;
-    k1a = a * hconf(a, fourpiG, c);
-    k2a = (a + k1a * dtau / 2.) * hconf(a + k1a * dtau / 2., fourpiG, c);
-    k3a = (a + k2a * dtau / 2.) * hconf(a + k2a * dtau / 2., fourpiG, c);
-    k4a = (a + k3a * dtau) * hconf(a + k3a * dtau, fourpiG, c);
-    a += dtau * (k1a + 2. * k2a + 2. * k3a + k4a) / 6.;
+    rk4bg_with(a, dtau, [&](double x) { return hconf(x, fourpiG, c); });
 }
 
 }  // namespace gev
@@ -163,10 +141,88 @@ GEV_EXPORT int64_t gev_ctx_footprint_bytes(const gev_box_desc *d) {
     return footprint(d->ngrid, d->npart, d->form == GEV_FORM_FP32);
 }
 
-GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
+namespace gev {
+
+// The device arrays of one box, each at an offset of one slab: a box's whole
+// state is one contiguous range, and the boxes of a batch lie one slab
+// stride apart (the offsets are those of box 0).  Offsets are 4 KiB aligned
+// (k rows are then 128-byte aligned too).
+struct SlabLayout {
+    std::vector<std::pair<void **, int64_t>> arrays;
+    int64_t bytes = 0;
+    void add(void **p, int64_t n) { arrays.push_back({p, n}); }
+    int64_t size() const {
+        int64_t off = 0;
+        for (auto &a : arrays) off = (off + a.second + 4095) / 4096 * 4096;
+        return (off + (2 << 20) - 1) / (2 << 20) * (2 << 20);  // slabs start on 2 MiB
+    }
+    void assign(char *base) const {
+        int64_t off = 0;
+        for (auto &a : arrays) {
+            *a.first = base + off;
+            off = (off + a.second + 4095) / 4096 * 4096;
+        }
+    }
+};
+
+static void slab_layout(gev_ctx *c, const Sizes &s, SlabLayout &L) {
+    const int64_t rb = rbytes(c), cb = cbytes(c);
+    L.add((void **)&c->phi, s.n3 * rb);
+    L.add((void **)&c->chi, s.n3 * rb);
+    L.add((void **)&c->source, s.n3 * rb);
+    L.add((void **)&c->Sij, 6 * s.n3 * rb);
+    L.add((void **)&c->Bi, 3 * s.n3 * rb);
+    L.add((void **)&c->scalarFT, s.nk * cb);
+    L.add((void **)&c->SijFT, 6 * s.nk * cb);
+    L.add((void **)&c->BiFT, 3 * s.nk * cb);
+    L.add((void **)&c->fft_tmp, s.nk * cb);
+    L.add((void **)&c->twiddle, c->N * 16);
+    if (c->fp32) L.add((void **)&c->twiddle_f, c->N * 8);
+    L.add((void **)&c->gridk2, c->N * 8);
+    L.add((void **)&c->kshift, c->N * 16);
+    L.add((void **)&c->pos, 3 * s.cap * 8);
+    L.add((void **)&c->q, 3 * s.cap * 8);
+    L.add((void **)&c->id, s.cap * 8);
+    L.add((void **)&c->pos2, 3 * s.cap * 8);
+    L.add((void **)&c->q2, 3 * s.cap * 8);
+    L.add((void **)&c->id2, s.cap * 8);
+    L.add((void **)&c->cell, s.cap * 4);
+    L.add((void **)&c->perm, s.cap * 4);
+    L.add((void **)&c->count, s.n3 * 4);
+    L.add((void **)&c->start, (s.n3 + 1) * 4);
+    L.add((void **)&c->scan_part, s.nblk_scan * 4);
+    L.add((void **)&c->sort_segs, (s.n3 / 256 + 1) * 4);
+    L.add((void **)&c->sort_big, sort_big_cap(s.cap) * 4);
+    L.add((void **)&c->sort_n, 8);
+    L.add((void **)&c->red_d, s.nred * 8);
+    L.add((void **)&c->red_u, 64);
+    c->pk_bins = spectrum_bins_cap(c->N);
+    L.add((void **)&c->pk_scratch, spectrum_scratch_doubles(c->N, c->pk_bins) * 8);
+}
+
+// points c's device arrays at the slab starting at base (a box of a batch)
+void assign_box_pointers(gev_ctx *c, char *base) {
+    SlabLayout L;
+    slab_layout(c, sizes_of(c->N, c->cap), L);
+    L.assign(base);
+}
+
+// bytes of one box's slab
+int64_t slab_bytes(int N, int64_t cap, bool fp32) {
+    gev_ctx c;
+    c.N = N;
+    c.fp32 = fp32;
+    SlabLayout L;
+    slab_layout(&c, sizes_of(N, cap), L);
+    return L.size();
+}
+
+// Creates a context holding nbox boxes of the same shape, one slab each; the
+// context's pointers are box 0's.
+int create_ctx(const gev_box_desc *d, int nbox, gev_ctx **out) {
     if (!out) return GEV_EINVAL;
     *out = nullptr;
-    if (!d || d->npart < 0 || d->npart > 0xFFFFFFFFll) return GEV_EINVAL;
+    if (!d || d->npart < 0 || d->npart > 0xFFFFFFFFll || nbox < 1) return GEV_EINVAL;
     if (d->form != GEV_FORM_FP64 && d->form != GEV_FORM_FP32) return GEV_EINVAL;
     if (!fft_size_supported(d->ngrid)) return GEV_EINVAL;
     gev_ctx *c = new (std::nothrow) gev_ctx();
@@ -186,46 +242,13 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
         c->own_stream = true;
     }
     Sizes s = sizes_of(c->N, c->cap);
-    bool ok = true;
-    auto A = [&](void **p, size_t bytes) {
-        if (!ok) return;
-        if (hipMalloc(p, bytes) != hipSuccess) {
-            *p = nullptr;
-            ok = false;
-        }
-    };
-    const int64_t rb = rbytes(c), cb = cbytes(c);
-    A((void **)&c->phi, s.n3 * rb);
-    A((void **)&c->chi, s.n3 * rb);
-    A((void **)&c->source, s.n3 * rb);
-    A((void **)&c->Sij, 6 * s.n3 * rb);
-    A((void **)&c->Bi, 3 * s.n3 * rb);
-    A((void **)&c->scalarFT, s.nk * cb);
-    A((void **)&c->SijFT, 6 * s.nk * cb);
-    A((void **)&c->BiFT, 3 * s.nk * cb);
-    A((void **)&c->fft_tmp, s.nk * cb);
-    A((void **)&c->twiddle, c->N * 16);
-    if (c->fp32) A((void **)&c->twiddle_f, c->N * 8);
-    A((void **)&c->gridk2, c->N * 8);
-    A((void **)&c->kshift, c->N * 16);
-    A((void **)&c->pos, 3 * s.cap * 8);
-    A((void **)&c->q, 3 * s.cap * 8);
-    A((void **)&c->id, s.cap * 8);
-    A((void **)&c->pos2, 3 * s.cap * 8);
-    A((void **)&c->q2, 3 * s.cap * 8);
-    A((void **)&c->id2, s.cap * 8);
-    A((void **)&c->cell, s.cap * 4);
-    A((void **)&c->perm, s.cap * 4);
-    A((void **)&c->count, s.n3 * 4);
-    A((void **)&c->start, (s.n3 + 1) * 4);
-    A((void **)&c->scan_part, s.nblk_scan * 4);
-    A((void **)&c->sort_segs, (s.n3 / 256 + 1) * 4);
-    A((void **)&c->sort_big, sort_big_cap(s.cap) * 4);
-    A((void **)&c->sort_n, 8);
-    A((void **)&c->red_d, s.nred * 8);
-    A((void **)&c->red_u, 64);
-    c->pk_bins = spectrum_bins_cap(c->N);
-    A((void **)&c->pk_scratch, spectrum_scratch_doubles(c->N, c->pk_bins) * 8);
+    SlabLayout L;
+    slab_layout(c, s, L);
+    c->nbox = nbox;
+    c->stride = L.size();
+    bool ok = hipMalloc(&c->slab, (size_t)c->stride * nbox) == hipSuccess;
+    if (!ok) c->slab = nullptr;
+    if (ok) L.assign((char *)c->slab);
     if (ok && hipHostMalloc((void **)&c->host_pinned, 64) != hipSuccess) ok = false;
     if (ok && hipHostMalloc((void **)&c->rec_pinned, sizeof(double) * 2 * REC_SLOTS) != hipSuccess) ok = false;
     if (!ok) {
@@ -239,25 +262,26 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     std::vector<double2> ks, tw;
     host_ktables(c->N, g, ks);
     host_twiddles(c->N, tw);
-    int rc = 0;
-    rc = rc ? rc : check_hip(c, hipMemcpy(c->gridk2, g.data(), c->N * 8, hipMemcpyHostToDevice), "k tables");
-    rc = rc ? rc : check_hip(c, hipMemcpy(c->kshift, ks.data(), c->N * 16, hipMemcpyHostToDevice), "k tables");
-    rc = rc ? rc : check_hip(c, hipMemcpy(c->twiddle, tw.data(), c->N * 16, hipMemcpyHostToDevice), "twiddles");
+    std::vector<float2> twf;
     if (c->fp32) {
-        std::vector<float2> twf(c->N);
+        twf.resize(c->N);
         for (int t = 0; t < c->N; t++) twf[t] = make_float2((float)tw[t].x, (float)tw[t].y);
-        rc = rc ? rc : check_hip(c, hipMemcpy(c->twiddle_f, twf.data(), c->N * 8, hipMemcpyHostToDevice), "twiddles");
     }
-    // every field starts at zero, like a fresh LATfield2 Field
-    const size_t zero_sets[][2] = {
-        {(size_t)c->phi, (size_t)(s.n3 * rb)},        {(size_t)c->chi, (size_t)(s.n3 * rb)},
-        {(size_t)c->source, (size_t)(s.n3 * rb)},     {(size_t)c->Sij, (size_t)(6 * s.n3 * rb)},
-        {(size_t)c->Bi, (size_t)(3 * s.n3 * rb)},     {(size_t)c->scalarFT, (size_t)(s.nk * cb)},
-        {(size_t)c->SijFT, (size_t)(6 * s.nk * cb)}, {(size_t)c->BiFT, (size_t)(3 * s.nk * cb)},
-        {(size_t)c->fft_tmp, (size_t)(s.nk * cb)}};
-    for (auto &z : zero_sets)
-        if (!rc) rc = launch_fill_zero(c, (void *)z[0], z[1]);
-    if (!rc) rc = launch_refile(c);
+    // every field starts at zero, like a fresh LATfield2 Field, and the filing
+    // is that of no particles: the whole slab of every box is cleared
+    int rc = launch_fill_zero(c, c->slab, (size_t)c->stride * nbox);
+    for (int b = 0; b < nbox && !rc; b++) {
+        rc = rc ? rc : check_hip(c, hipMemcpy(box_ptr(c->gridk2, c->stride, b), g.data(), c->N * 8,
+                                              hipMemcpyHostToDevice), "k tables");
+        rc = rc ? rc : check_hip(c, hipMemcpy(box_ptr(c->kshift, c->stride, b), ks.data(), c->N * 16,
+                                              hipMemcpyHostToDevice), "k tables");
+        rc = rc ? rc : check_hip(c, hipMemcpy(box_ptr(c->twiddle, c->stride, b), tw.data(), c->N * 16,
+                                              hipMemcpyHostToDevice), "twiddles");
+        if (c->fp32)
+            rc = rc ? rc : check_hip(c, hipMemcpy(box_ptr(c->twiddle_f, c->stride, b), twf.data(), c->N * 8,
+                                                  hipMemcpyHostToDevice), "twiddles");
+    }
+    if (!rc) c->count_clean = true;
     if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "create sync");
     if (rc) {
         free_all(c);
@@ -269,8 +293,13 @@ GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) {
     return 0;
 }
 
+}  // namespace gev
+
+GEV_EXPORT int gev_ctx_create(const gev_box_desc *d, gev_ctx **out) { return create_ctx(d, 1, out); }
+
 GEV_EXPORT int gev_ctx_destroy(gev_ctx *c) {
     if (!c) return GEV_EINVAL;
+    if (c->view) return fail(c, GEV_ESTATE, "a box of a batch is freed with its batch (gev_batch_destroy)");
     (void)hipStreamSynchronize(c->stream);
     for (auto &r : c->pending) {
         (void)hipEventDestroy(r.beg);
@@ -820,7 +849,7 @@ GEV_EXPORT int gev_run_cycles(gev_ctx *c, const gev_sim_params *p, gev_cycle_sta
 
 // ---------------------------------------------------------------- outputs
 
-namespace {
+namespace gev {
 
 // A number as writePowerSpectrum prints it ("%e" in the table, "%f" for the
 // header redshift) and its interpolating pass reads it back (fscanf %le / %lf),
@@ -831,12 +860,21 @@ double text_round_trip(const char *fmt, double v) {
     return strtod(buf, nullptr);
 }
 
-struct PkWriter {
-    int numbins;
-    double rescalek, rescalep;
-    std::vector<double> kb, pw, ks, ps, p2, pre, pre_z;
-    std::vector<int32_t> oc, have;
-};
+void PkWriter::init(const gev_ctx *c, const gev_sim_params *p, int num_pk_, int numbins_, double boxsize) {
+    const gev_cosmology *cs = &p->cosmo;
+    const double n3d = (double)c->N * (double)c->N * (double)c->N;
+    // rescalep of the delta spectrum: output.hpp:2026 (GR, no ncdm), 1853 (Newton)
+    const double om = p->gr_flag > 0 ? cs->Omega_cdm + cs->Omega_b + 0.0 : cs->Omega_m;
+    num_pk = num_pk_;
+    numbins = numbins_;
+    rescalek = boxsize;
+    rescalep = n3d * n3d * 2. * M_PI * M_PI * om * om;
+    for (auto *v : {&kb, &pw, &ks, &ps, &p2}) v->assign(numbins, 0.);
+    pre.assign((size_t)num_pk * numbins, 0.);
+    pre_z.assign(num_pk, 0.);
+    oc.assign(numbins, 0);
+    have.assign(num_pk, 0);
+}
 
 // writeSpectra's delta spectrum (output.hpp:1820-1827 Newton, 2001-2010 GR)
 // and writePowerSpectrum (tools.hpp:268-346, EXACT_OUTPUT_REDSHIFTS) of
@@ -844,6 +882,7 @@ struct PkWriter {
 int write_spectrum(gev_ctx *c, const gev_sim_params *p, const gev_cycle_state *st, PkWriter &w, int j,
                    double zt, double *k, double *pk, double *sk, double *spk, int32_t *occ,
                    gev_pk_output_info *info) {
+    if (w.numbins != w.kb.size()) return fail(c, GEV_EINVAL, "spectrum writer not set up");
     const int nb = w.numbins;
     int rc = gev_projection_init(c, GEV_SOURCE);
     if (!rc) rc = p->gr_flag > 0 ? gev_projection_T00_project(c, st->a, 1, 1.0)
@@ -882,7 +921,7 @@ int write_spectrum(gev_ctx *c, const gev_sim_params *p, const gev_cycle_state *s
     return 0;
 }
 
-}  // namespace
+}  // namespace gev
 
 GEV_EXPORT int gev_run_to_outputs(gev_ctx *c, const gev_sim_params *p, gev_cycle_state *st, int32_t num_pk,
                                   const double *z_pk, int32_t numbins, double boxsize, int32_t max_cycles,
@@ -898,18 +937,8 @@ GEV_EXPORT int gev_run_to_outputs(gev_ctx *c, const gev_sim_params *p, gev_cycle
     for (int j = 1; j < num_pk; j++)
         if (!(z_pk[j] < z_pk[j - 1])) return fail(c, GEV_EINVAL, "z_pk must be strictly descending");
     const gev_cosmology *cs = &p->cosmo;
-    const double n3d = (double)c->N * (double)c->N * (double)c->N;
-    // rescalep of the delta spectrum: output.hpp:2026 (GR, no ncdm), 1853 (Newton)
-    const double om = p->gr_flag > 0 ? cs->Omega_cdm + cs->Omega_b + 0.0 : cs->Omega_m;
     PkWriter w;
-    w.numbins = numbins;
-    w.rescalek = boxsize;
-    w.rescalep = n3d * n3d * 2. * M_PI * M_PI * om * om;
-    for (auto *v : {&w.kb, &w.pw, &w.ks, &w.ps, &w.p2}) v->assign(numbins, 0.);
-    w.pre.assign((size_t)num_pk * numbins, 0.);
-    w.pre_z.assign(num_pk, 0.);
-    w.oc.assign(numbins, 0);
-    w.have.assign(num_pk, 0);
+    w.init(c, p, num_pk, numbins, boxsize);
     int pkcount = 0, ran = 0, rc = 0;
     // a run stopped at an output resumes at that cycle's look-ahead check
     bool resume = st->stage == 1;

@@ -388,13 +388,35 @@ __device__ __forceinline__ void add_records(Partial<R> &P, const ST &st, const T
 
 // FULL: N is a multiple of the tile, so the tile and source-region extents
 // are compile-time constants (index arithmetic without integer division).
-template <int KIND, bool USE_PHI, bool ACCUM, bool POW2, bool FULL, typename F>
+// BATCH: the launch covers the boxes of a batch (its own instantiation, so
+// the one-box kernel is unchanged).
+template <int KIND, bool USE_PHI, bool ACCUM, bool POW2, bool FULL, typename F, bool BATCH = false>
 __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const double *__restrict__ pos, const double *__restrict__ q, int64_t cap,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
     const F *__restrict__ phi, F *__restrict__ out_s, F *__restrict__ out_t, int N,
-    int ntx, int nty, int ntz, double a, double mass_s, double mass_t) {
+    int ntx, int nty, int ntz, double a, double mass_s, double mass_t, const double *__restrict__ pos_b,
+    const double *__restrict__ q_b, gev::Batch bt) {
     using K = Kind<KIND>;
+    if constexpr (BATCH) {
+        // box blockIdx.y of a batch: its arrays, live particle set and coefficients
+        const int b = blockIdx.y;
+        if (!gev::box_on(bt, b)) return;
+        const gev::BoxCycle &bc = bt.cyc[b];
+        if (bc.parity) {
+            pos = pos_b;
+            q = q_b;
+        }
+        pos = gev::box_ptr(pos, bt.stride, b);
+        q = gev::box_ptr(q, bt.stride, b);
+        start = gev::box_ptr(start, bt.stride, b);
+        perm = gev::box_ptr(perm, bt.stride, b);
+        phi = gev::box_ptr(phi, bt.stride, b);
+        out_s = gev::box_ptr(out_s, bt.stride, b);
+        out_t = gev::box_ptr(out_t, bt.stride, b);
+        a = KIND == 5 ? 1.0 : bc.a;
+        mass_s = mass_t = KIND == 5 ? bc.dep_m : bc.dep_m_a;
+    }
     constexpr int CAP = dep_cap<F>();
     // the records, or (FP32 form, dense tiles) the fixed-point accumulators
     // and the target values
@@ -919,37 +941,52 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
 }
 
 template <int KIND, bool USE_PHI, bool ACCUM, typename F>
-void launch_t(gev_ctx *ctx, F *out_s, F *out_t, double a, double ms, double mt) {
+void launch_t(gev_ctx *ctx, F *out_s, F *out_t, double a, double ms, double mt, const gev::Batch &bt) {
     const int N = ctx->N;
     const int ntx = (N + TX - 1) / TX, nty = (N + TY - 1) / TY, ntz = (N + TZ - 1) / TZ;
     const int64_t nblk = (int64_t)ntx * nty * ntz;
+    const unsigned nbox = bt.cyc ? (unsigned)ctx->nbox : 1u;
     const bool pow2 = (N & (N - 1)) == 0, full = N % TX == 0 && N % TY == 0 && N % TZ == 0;
-#define GEV_DEP_LAUNCH(P2, FL)                                                                              \
-    hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, P2, FL, F>), dim3((unsigned)nblk), dim3(TPB), 0, \
-                       ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, gev::fld<F>(ctx->phi), out_s, \
-                       out_t, N, ntx, nty, ntz, a, ms, mt)
-    if (pow2 && full) GEV_DEP_LAUNCH(true, true);
-    else if (full) GEV_DEP_LAUNCH(false, true);
-    else GEV_DEP_LAUNCH(false, false);
+#define GEV_DEP_LAUNCH(P2, FL, B)                                                                             \
+    hipLaunchKernelGGL((k_deposit_tiled<KIND, USE_PHI, ACCUM, P2, FL, F, B>), dim3((unsigned)nblk, nbox), dim3(TPB), \
+                       0, ctx->stream, ctx->pos, ctx->q, ctx->cap, ctx->start, ctx->perm, gev::fld<F>(ctx->phi),     \
+                       out_s, out_t, N, ntx, nty, ntz, a, ms, mt, ctx->pos2, ctx->q2, bt)
+    if (bt.cyc) {
+        // the batched cycle deposits in store mode only
+        if constexpr (!ACCUM) {
+            if (pow2 && full) GEV_DEP_LAUNCH(true, true, true);
+            else if (full) GEV_DEP_LAUNCH(false, true, true);
+            else GEV_DEP_LAUNCH(false, false, true);
+        }
+    } else if (pow2 && full) {
+        GEV_DEP_LAUNCH(true, true, false);
+    } else if (full) {
+        GEV_DEP_LAUNCH(false, true, false);
+    } else {
+        GEV_DEP_LAUNCH(false, false, false);
+    }
 #undef GEV_DEP_LAUNCH
 }
 
 template <int KIND, typename F>
-void launch_f(gev_ctx *ctx, bool use_phi, bool accum, F *out_s, F *out_t, double a, double ms, double mt) {
+void launch_f(gev_ctx *ctx, bool use_phi, bool accum, F *out_s, F *out_t, double a, double ms, double mt,
+              const gev::Batch &bt) {
     if (use_phi) {
-        if (accum) launch_t<KIND, true, true>(ctx, out_s, out_t, a, ms, mt);
-        else launch_t<KIND, true, false>(ctx, out_s, out_t, a, ms, mt);
+        if (accum) launch_t<KIND, true, true>(ctx, out_s, out_t, a, ms, mt, bt);
+        else launch_t<KIND, true, false>(ctx, out_s, out_t, a, ms, mt, bt);
     } else {
-        if (accum) launch_t<KIND, false, true>(ctx, out_s, out_t, a, ms, mt);
-        else launch_t<KIND, false, false>(ctx, out_s, out_t, a, ms, mt);
+        if (accum) launch_t<KIND, false, true>(ctx, out_s, out_t, a, ms, mt, bt);
+        else launch_t<KIND, false, false>(ctx, out_s, out_t, a, ms, mt, bt);
     }
 }
 
 template <int KIND>
 void launch_k(gev_ctx *ctx, bool use_phi, bool accum, double *out_s, double *out_t, double a, double ms,
-              double mt) {
-    if (ctx->fp32) launch_f<KIND>(ctx, use_phi, accum, gev::fld<float>(out_s), gev::fld<float>(out_t), a, ms, mt);
-    else launch_f<KIND>(ctx, use_phi, accum, out_s, out_t, a, ms, mt);
+              double mt, const gev::Batch &bt = gev::Batch{}) {
+    if (ctx->fp32)
+        launch_f<KIND>(ctx, use_phi, accum, gev::fld<float>(out_s), gev::fld<float>(out_t), a, ms, mt, bt);
+    else
+        launch_f<KIND>(ctx, use_phi, accum, out_s, out_t, a, ms, mt, bt);
 }
 
 }  // namespace
@@ -999,6 +1036,24 @@ int launch_deposit(gev_ctx *ctx, int kind, double a, int use_phi, double coeff) 
     }
     default:
         return fail(ctx, GEV_EINVAL, "unknown deposit kind");
+    }
+    return check_hip(ctx, hipGetLastError(), "deposit launch");
+}
+
+// The field stage's deposits over the boxes of a batch (store mode; every
+// box's coefficients from its BoxCycle): T00 + Tij (GR) or the CIC mass and
+// Tij (Newton).
+int launch_deposit_batch(gev_ctx *ctx, bool gr, const Batch &bt) {
+    if (gr) {
+        ScopedTimer t(ctx, "projection_T00_Tij_project");
+        launch_k<3>(ctx, true, false, ctx->source, ctx->Sij, 0., 0., 0., bt);
+    } else {
+        {
+            ScopedTimer t(ctx, "scalar_projection_cic_project");
+            launch_k<5>(ctx, false, false, ctx->source, nullptr, 1., 0., 0., bt);
+        }
+        ScopedTimer t(ctx, "projection_Tij_project");
+        launch_k<2>(ctx, true, false, nullptr, ctx->Sij, 0., 0., 0., bt);
     }
     return check_hip(ctx, hipGetLastError(), "deposit launch");
 }

@@ -37,9 +37,10 @@ __global__ __launch_bounds__(TPB) void k_r2c_x(const T *__restrict__ in,
                                                typename cplx_of<T>::type *__restrict__ out, int P,
                                                int64_t nrows,
                                                const typename cplx_of<T>::type *__restrict__ twN,
-                                               int64_t in_cs, int64_t out_cs) {
+                                               int64_t in_cs, int64_t out_cs, gev::Batch bt) {
     using C = typename cplx_of<T>::type;
-    in += blockIdx.z * in_cs;  // component blockIdx.z
+    if (!gev::box_on(bt, blockIdx.z)) return;
+    in += blockIdx.z * in_cs;  // component (or box of a batch) blockIdx.z
     out += blockIdx.z * out_cs;
     constexpr int M = N / 2;
     constexpr int LINES = x_lines<N>();
@@ -87,9 +88,10 @@ template <typename T, int N>
 __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *__restrict__ in,
                                                T *__restrict__ out, int P, int64_t nrows,
                                                const typename cplx_of<T>::type *__restrict__ twN,
-                                               int64_t in_cs, int64_t out_cs) {
+                                               int64_t in_cs, int64_t out_cs, gev::Batch bt) {
     using C = typename cplx_of<T>::type;
-    in += blockIdx.z * in_cs;  // component blockIdx.z
+    if (!gev::box_on(bt, blockIdx.z)) return;
+    in += blockIdx.z * in_cs;  // component (or box of a batch) blockIdx.z
     out += blockIdx.z * out_cs;
     constexpr int M = N / 2;
     constexpr int LINES = x_lines<N>();
@@ -147,10 +149,11 @@ __global__ __launch_bounds__(col_tpb<T>()) void k_c2c_cols(const typename cplx_o
                                                   typename cplx_of<T>::type *out, int H,
                                                   int64_t es, int64_t os,
                                                   const typename cplx_of<T>::type *__restrict__ twN,
-                                                  int64_t cs) {
+                                                  int64_t cs, gev::Batch bt) {
     using C = typename cplx_of<T>::type;
     constexpr int TPB = col_tpb<T>();
-    in += blockIdx.z * cs;  // component blockIdx.z
+    if (!gev::box_on(bt, blockIdx.z)) return;
+    in += blockIdx.z * cs;  // component (or box of a batch) blockIdx.z
     out += blockIdx.z * cs;
     constexpr int W = col_tile<T, N>();
     __shared__ C buf[N * W];
@@ -202,33 +205,45 @@ const char *pass_name(int pass, int n) {
 }
 
 // passes: bit 0 = x (r2c), bit 1 = y, bit 2 = z, all components pass by pass.
-// The x pass is one launch over up to MAXC components (blockIdx.z).
+// The x pass is one launch over up to MAXC components (blockIdx.z).  Over the
+// boxes of a batch (bt.cyc set) every pass runs one launch per component with
+// the box in blockIdx.z.
 template <typename T, int N>
 int run_forward(gev_ctx *ctx, const T *real, typename cplx_of<T>::type *k, int ncomp,
-                const typename cplx_of<T>::type *tw, int passes) {
+                const typename cplx_of<T>::type *tw, int passes, const gev::Batch &bt) {
+    using C = typename cplx_of<T>::type;
     const int P = ctx->P, H = N / 2 + 1;
     const int64_t nrows = (int64_t)N * N;
     const int64_t n3v = (int64_t)N * N * N, nkv = (int64_t)P * N * N;
     constexpr int LINES = x_lines<N>();
     constexpr int W = col_tile<T, N>();
+    const bool batch = bt.cyc != nullptr;
+    const unsigned nb = batch ? (unsigned)ctx->nbox : 1u;
+    const int64_t rs = batch ? ctx->stride / (int64_t)sizeof(T) : 0, ks = batch ? ctx->stride / (int64_t)sizeof(C) : 0;
     for (int c0 = 0; c0 < ncomp; c0 += MAXC) {
         const int nc = ncomp - c0 < MAXC ? ncomp - c0 : MAXC;
         if (passes & 1) {
-            gev::ScopedTimer t(ctx, pass_name(P_R2C_X, nc));
-            hipLaunchKernelGGL((k_r2c_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nc), dim3(TPB), 0,
-                               ctx->stream, real + c0 * n3v, k + c0 * nkv, P, nrows, tw, n3v, nkv);
+            gev::ScopedTimer t(ctx, pass_name(P_R2C_X, batch ? 1 : nc));
+            if (batch) {
+                for (int c = c0; c < c0 + nc; c++)
+                    hipLaunchKernelGGL((k_r2c_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nb), dim3(TPB), 0,
+                                       ctx->stream, real + c * n3v, k + c * nkv, P, nrows, tw, rs, ks, bt);
+            } else {
+                hipLaunchKernelGGL((k_r2c_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nc), dim3(TPB), 0,
+                                   ctx->stream, real + c0 * n3v, k + c0 * nkv, P, nrows, tw, n3v, nkv, bt);
+            }
         }
         // the column passes run one launch per component (measured faster
         // than one launch over the components)
         for (int c = c0; c < c0 + nc && (passes & 2); c++) {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Y, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0, ctx->stream,
-                               k + c * nkv, k + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
+            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
+                               ctx->stream, k + c * nkv, k + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
         }
         for (int c = c0; c < c0 + nc && (passes & 4); c++) {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Z, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0, ctx->stream,
-                               k + c * nkv, k + c * nkv, H, (int64_t)P * N, (int64_t)P, tw, (int64_t)0);
+            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
+                               ctx->stream, k + c * nkv, k + c * nkv, H, (int64_t)P * N, (int64_t)P, tw, ks, bt);
         }
     }
     return gev::check_hip(ctx, hipGetLastError(), "fft forward launch");
@@ -239,25 +254,36 @@ int run_forward(gev_ctx *ctx, const T *real, typename cplx_of<T>::type *k, int n
 // components.  Otherwise every component goes z, y, x through tmp.
 template <typename T, int N>
 int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int ncomp,
-                 typename cplx_of<T>::type *tmp, const typename cplx_of<T>::type *tw, bool from_y) {
+                 typename cplx_of<T>::type *tmp, const typename cplx_of<T>::type *tw, bool from_y,
+                 const gev::Batch &bt) {
+    using C = typename cplx_of<T>::type;
     const int P = ctx->P, H = N / 2 + 1;
     const int64_t nrows = (int64_t)N * N;
     const int64_t n3v = (int64_t)N * N * N, nkv = (int64_t)P * N * N;
     constexpr int LINES = x_lines<N>();
     constexpr int W = col_tile<T, N>();
+    const bool batch = bt.cyc != nullptr;
+    const unsigned nb = batch ? (unsigned)ctx->nbox : 1u;
+    const int64_t rs = batch ? ctx->stride / (int64_t)sizeof(T) : 0, ks = batch ? ctx->stride / (int64_t)sizeof(C) : 0;
     if (from_y) {
-        auto *kk = const_cast<typename cplx_of<T>::type *>(k);
+        auto *kk = const_cast<C *>(k);
         for (int c0 = 0; c0 < ncomp; c0 += MAXC) {
             const int nc = ncomp - c0 < MAXC ? ncomp - c0 : MAXC;
             for (int c = c0; c < c0 + nc; c++) {
                 gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
-                hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0, ctx->stream,
-                                   kk + c * nkv, kk + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
+                hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
+                                   ctx->stream, kk + c * nkv, kk + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
             }
             {
-                gev::ScopedTimer t(ctx, pass_name(P_C2R_X, nc));
-                hipLaunchKernelGGL((k_c2r_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nc), dim3(TPB), 0,
-                                   ctx->stream, kk + c0 * nkv, real + c0 * n3v, P, nrows, tw, nkv, n3v);
+                gev::ScopedTimer t(ctx, pass_name(P_C2R_X, batch ? 1 : nc));
+                if (batch) {
+                    for (int c = c0; c < c0 + nc; c++)
+                        hipLaunchKernelGGL((k_c2r_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nb), dim3(TPB), 0,
+                                           ctx->stream, kk + c * nkv, real + c * n3v, P, nrows, tw, ks, rs, bt);
+                } else {
+                    hipLaunchKernelGGL((k_c2r_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nc), dim3(TPB), 0,
+                                       ctx->stream, kk + c0 * nkv, real + c0 * n3v, P, nrows, tw, nkv, n3v, bt);
+                }
             }
         }
         return gev::check_hip(ctx, hipGetLastError(), "fft backward launch");
@@ -265,18 +291,18 @@ int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int 
     for (int c = 0; c < ncomp; c++) {
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Z_INV, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0,
-                               ctx->stream, k + c * nkv, tmp, H, (int64_t)P * N, (int64_t)P, tw, (int64_t)0);
+            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
+                               ctx->stream, k + c * nkv, tmp, H, (int64_t)P * N, (int64_t)P, tw, ks, bt);
         }
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N), dim3(col_tpb<T>()), 0,
-                               ctx->stream, tmp, tmp, H, (int64_t)P, (int64_t)P * N, tw, (int64_t)0);
+            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
+                               ctx->stream, tmp, tmp, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
         }
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2R_X, 1));
-            hipLaunchKernelGGL((k_c2r_x<T, N>), dim3(gev::grid_for(nrows, LINES)), dim3(TPB), 0,
-                               ctx->stream, tmp, real + c * n3v, P, nrows, tw, (int64_t)0, (int64_t)0);
+            hipLaunchKernelGGL((k_c2r_x<T, N>), dim3(gev::grid_for(nrows, LINES), 1, nb), dim3(TPB), 0,
+                               ctx->stream, tmp, real + c * n3v, P, nrows, tw, ks, rs, bt);
         }
     }
     return gev::check_hip(ctx, hipGetLastError(), "fft backward launch");
@@ -299,13 +325,14 @@ bool fft_size_supported(int N) {
 }
 
 // the passes in the context's form: FP64, or FP32 (float fields, float2 k fields)
-static int forward(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp, int passes) {
+static int forward(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp, int passes,
+                   const Batch &bt = Batch{}) {
     switch (ctx->N) {
 #define CASE(n)                                                                                                  \
     case n:                                                                                                      \
         return ctx->fp32 ? run_forward<float, n>(ctx, fld<float>(real), cfld<float2>(kfield), ncomp, ctx->twiddle_f, \
-                                                 passes)                                                         \
-                         : run_forward<double, n>(ctx, real, kfield, ncomp, ctx->twiddle, passes);
+                                                 passes, bt)                                                     \
+                         : run_forward<double, n>(ctx, real, kfield, ncomp, ctx->twiddle, passes, bt);
         GEV_FFT_SIZES(CASE)
 #undef CASE
     default:
@@ -313,13 +340,14 @@ static int forward(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp,
     }
 }
 
-static int backward(gev_ctx *ctx, const double2 *kfield, double *real, int ncomp, bool from_y) {
+static int backward(gev_ctx *ctx, const double2 *kfield, double *real, int ncomp, bool from_y,
+                    const Batch &bt = Batch{}) {
     switch (ctx->N) {
 #define CASE(n)                                                                                             \
     case n:                                                                                                 \
         return ctx->fp32 ? run_backward<float, n>(ctx, cfld<float2>(kfield), fld<float>(real), ncomp,       \
-                                                  cfld<float2>(ctx->fft_tmp), ctx->twiddle_f, from_y)       \
-                         : run_backward<double, n>(ctx, kfield, real, ncomp, ctx->fft_tmp, ctx->twiddle, from_y);
+                                                  cfld<float2>(ctx->fft_tmp), ctx->twiddle_f, from_y, bt)   \
+                         : run_backward<double, n>(ctx, kfield, real, ncomp, ctx->fft_tmp, ctx->twiddle, from_y, bt);
         GEV_FFT_SIZES(CASE)
 #undef CASE
     default:
@@ -335,16 +363,16 @@ int fft_backward(gev_ctx *ctx, const double2 *kfield, double *real, int ncomp) {
     return backward(ctx, kfield, real, ncomp, false);
 }
 
-int fft_forward_xy(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp) {
-    return forward(ctx, real, kfield, ncomp, 3);
+int fft_forward_xy(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp, const Batch &bt) {
+    return forward(ctx, real, kfield, ncomp, 3, bt);
 }
 
-int fft_forward_y(gev_ctx *ctx, double2 *kfield, int ncomp) {
-    return forward(ctx, nullptr, kfield, ncomp, 2);
+int fft_forward_y(gev_ctx *ctx, double2 *kfield, int ncomp, const Batch &bt) {
+    return forward(ctx, nullptr, kfield, ncomp, 2, bt);
 }
 
-int fft_backward_yx(gev_ctx *ctx, double2 *kfield, double *real, int ncomp) {
-    return backward(ctx, kfield, real, ncomp, true);
+int fft_backward_yx(gev_ctx *ctx, double2 *kfield, double *real, int ncomp, const Batch &bt) {
+    return backward(ctx, kfield, real, ncomp, true, bt);
 }
 
 }  // namespace gev

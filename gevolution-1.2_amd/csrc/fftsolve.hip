@@ -113,6 +113,7 @@ template <typename C> struct ZArgsT {
     int64_t nkv;         // complexes per component
     int P, H;
     double c0, c1;       // Poisson: coeff/(-N^3), modif; tensor: a^2 dtau; vector: coeff, modif
+    gev::Batch bt;       // over the boxes of a batch: box blockIdx.z, coefficients from its BoxCycle
 };
 using ZArgs = ZArgsT<double2>;
 
@@ -126,6 +127,22 @@ __global__ __launch_bounds__((zs_threads<N, KIND, C>())) void k_zsolve(ZArgsT<C>
     static_assert(NT % W == 0, "a thread keeps one column");
     __shared__ C buf[N * W];
     __shared__ C tw[N];
+    if (A.bt.cyc) {
+        const int b = blockIdx.z;
+        if (!gev::box_on(A.bt, b)) return;
+        const gev::BoxCycle &bc = A.bt.cyc[b];
+        A.in = gev::box_ptr(A.in, A.bt.stride, b);
+        A.out = gev::box_ptr(A.out, A.bt.stride, b);
+        A.bift = gev::box_ptr(A.bift, A.bt.stride, b);
+        A.kstore = gev::box_ptr(A.kstore, A.bt.stride, b);
+        A.k0 = gev::box_ptr(A.k0, A.bt.stride, b);
+        if (KIND == ZS_POISSON) {
+            A.c0 = bc.pois_c0;
+            A.c1 = bc.pois_c1;
+        } else if (KIND == ZS_TENSOR) {
+            A.c0 = bc.a2dtau;
+        }
+    }
 
     // XCD-aware tile order: hardware workgroup ids go round-robin over the 8
     // XCDs, so tiles l and l+1 (which share 128-byte rows) are given to ids
@@ -326,6 +343,7 @@ template <typename F, typename C> struct XArgsT {
     int P;
     int64_t nrows, n3v, nkv;
     double c0, c1, c2, c3;  // tensor: coeff; scalar: bg, coeff, coeff2, coeff3
+    gev::Batch bt;          // over the boxes of a batch: box blockIdx.y, coefficients from its BoxCycle
 };
 using XArgs = XArgsT<double, double2>;
 
@@ -351,6 +369,23 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgsT<F, C> A) {
     constexpr bool EXACT = PER * TPB == LINES * M;
     __shared__ C buf[LINES * xstride<M>()];
     __shared__ C tw[M];
+    if (A.bt.cyc) {
+        const int b = blockIdx.y;
+        if (!gev::box_on(A.bt, b)) return;
+        const gev::BoxCycle &bc = A.bt.cyc[b];
+        A.phi = gev::box_ptr(A.phi, A.bt.stride, b);
+        A.chi = gev::box_ptr(A.chi, A.bt.stride, b);
+        A.src = gev::box_ptr(A.src, A.bt.stride, b);
+        A.out = gev::box_ptr(A.out, A.bt.stride, b);
+        if (NC == 6) {
+            A.c0 = bc.tens_c;
+        } else {
+            A.c0 = bc.prep_bg;
+            A.c1 = bc.prep_c;
+            A.c2 = bc.prep_c2;
+            A.c3 = bc.prep_c3;
+        }
+    }
     const int64_t row0 = (int64_t)blockIdx.x * LINES;
     for (int t = threadIdx.x; t < M; t += TPB) tw[t] = A.tw[2 * t];
     C v[NC][PER];
@@ -421,8 +456,9 @@ __global__ __launch_bounds__(256) void k_r2c_x_prep(XArgsT<F, C> A) {
 
 template <int N, int NC, typename F, typename C> int launch_x(gev_ctx *c, const XArgsT<F, C> &A) {
     constexpr int LINES = xprep_lines<N, NC>();
-    hipLaunchKernelGGL((k_r2c_x_prep<N, NC, F, C>), dim3(gev::grid_for(A.nrows, LINES)), dim3(256), 0, c->stream,
-                       A);
+    const unsigned nbox = A.bt.cyc ? (unsigned)c->nbox : 1u;
+    hipLaunchKernelGGL((k_r2c_x_prep<N, NC, F, C>), dim3(gev::grid_for(A.nrows, LINES), nbox), dim3(256), 0,
+                       c->stream, A);
     return gev::check_hip(c, hipGetLastError(), "fused x pass launch");
 }
 
@@ -443,6 +479,7 @@ template <typename F, typename C> XArgsT<F, C> xargs_as(const XArgs &A) {
     B.c1 = A.c1;
     B.c2 = A.c2;
     B.c3 = A.c3;
+    B.bt = A.bt;
     return B;
 }
 
@@ -467,8 +504,9 @@ template <int NC> int launch_x_nc(gev_ctx *c, XArgs A) {
 template <int N, int KIND, typename C> int launch_n(gev_ctx *c, const ZArgsT<C> &A) {
     constexpr int W = zs_cols<N, KIND, C>();
     const int H = N / 2 + 1;
-    hipLaunchKernelGGL((k_zsolve<N, KIND, C>), dim3((H + W - 1) / W, N), dim3(zs_threads<N, KIND, C>()), 0, c->stream,
-                       A);
+    const unsigned nbox = A.bt.cyc ? (unsigned)c->nbox : 1u;
+    hipLaunchKernelGGL((k_zsolve<N, KIND, C>), dim3((H + W - 1) / W, N, nbox), dim3(zs_threads<N, KIND, C>()), 0,
+                       c->stream, A);
     return gev::check_hip(c, hipGetLastError(), "fused z solve launch");
 }
 
@@ -487,6 +525,7 @@ template <typename C> ZArgsT<C> zargs_as(const ZArgs &A) {
     B.H = A.H;
     B.c0 = A.c0;
     B.c1 = A.c1;
+    B.bt = A.bt;
     return B;
 }
 
@@ -512,9 +551,10 @@ template <int KIND> int launch_kind(gev_ctx *c, ZArgs A) {
 namespace gev {
 
 int launch_zsolve_poisson(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
-                          double2 *kstore, double2 *k0) {
+                          double2 *kstore, double2 *k0, const Batch &bt) {
     gev::ScopedTimer t(c, "fft_z_poisson_z");
     ZArgs A{};
+    A.bt = bt;
     A.in = in;
     A.out = out;
     A.kstore = kstore;
@@ -525,9 +565,10 @@ int launch_zsolve_poisson(gev_ctx *c, const double2 *in, double2 *out, double co
 }
 
 int launch_zsolve_tensor(gev_ctx *c, const double2 *in, double2 *out, double a2dtau, bool evolve,
-                         double2 *kstore, double2 *k0) {
+                         double2 *kstore, double2 *k0, const Batch &bt) {
     gev::ScopedTimer t(c, evolve ? "fft_z_tensor_z" : "fft_z_tensor_scalar_z");
     ZArgs A{};
+    A.bt = bt;
     A.in = in;
     A.out = out;
     A.bift = c->BiFT;
@@ -537,9 +578,10 @@ int launch_zsolve_tensor(gev_ctx *c, const double2 *in, double2 *out, double a2d
     return evolve ? launch_kind<ZS_TENSOR>(c, A) : launch_kind<ZS_TENSOR_SCALAR>(c, A);
 }
 
-int launch_x_prep_tensor(gev_ctx *c, double coeff) {
+int launch_x_prep_tensor(gev_ctx *c, double coeff, const Batch &bt) {
     ScopedTimer t(c, "prepare_tensor_fft_r2c_x");
     XArgs A{};
+    A.bt = bt;
     A.phi = c->phi;
     A.src = c->Sij;
     A.out = c->SijFT;
@@ -547,9 +589,10 @@ int launch_x_prep_tensor(gev_ctx *c, double coeff) {
     return launch_x_nc<6>(c, A);
 }
 
-int launch_x_prep_scalar(gev_ctx *c, double bg, double coeff, double coeff2, double coeff3) {
+int launch_x_prep_scalar(gev_ctx *c, double bg, double coeff, double coeff2, double coeff3, const Batch &bt) {
     ScopedTimer t(c, "prepare_scalar_fft_r2c_x");
     XArgs A{};
+    A.bt = bt;
     A.phi = c->phi;
     A.chi = c->chi;
     A.src = c->source;

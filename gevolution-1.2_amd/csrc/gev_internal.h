@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gev_pm.h"
@@ -32,9 +33,91 @@ struct TimerAcc {
     double ms = 0.0;
 };
 
+// ---------------------------------------------------------------- background
+
+// Conformal Hubble rate, background.hpp:137-140 (no ncdm), the same code on
+// the host and the device (-ffp-contract=off on both, IEEE sqrt and division),
+// so a, tau and dtau advanced on the device equal the host's bit for bit.  The
+// dark-energy fluid term is skipped when Omega_fld is zero: it is then exactly
+// zero and adding it changes nothing, and the device's exp / pow need not
+// equal the host library's.
+__host__ __device__ inline double hconf_of(double a, double fourpiG, double Om_cdm, double Om_b, double Om_L,
+                                           double Om_rad, double Om_fld, double w0, double wa) {
+    double s = ((Om_cdm + Om_b + 0.0) / a) + (Om_L * a * a) + (Om_rad / a / a);
+    if (Om_fld != 0.0) s = s + (Om_fld * exp(3. * wa * (a - 1.)) / pow(a, 1. + 3. * (w0 + wa)));
+    return sqrt((2. * fourpiG / 3.) * s);
+}
+
+// rungekutta4bg, background.hpp:167-177, over a Hubble-rate function of a
+template <typename H> __host__ __device__ inline void rk4bg_with(double &a, double dtau, const H &hconf) {
+    double k1a, k2a, k3a, k4a;
+    k1a = a * hconf(a);
+    k2a = (a + k1a * dtau / 2.) * hconf(a + k1a * dtau / 2.);
+    k3a = (a + k2a * dtau / 2.) * hconf(a + k2a * dtau / 2.);
+    k4a = (a + k3a * dtau) * hconf(a + k3a * dtau);
+    a += dtau * (k1a + 2. * k2a + 2. * k3a + k4a) / 6.;
+}
+
+// ---------------------------------------------------------------- batches
+
+// Scalars of one box of a batch for one cycle (device array, one per box).
+// The state and the static parameters are set by the host; k_cycle_begin
+// derives the cycle's coefficients with the host driver's expressions and
+// k_cycle_end advances the state (main.cpp:766-874).
+struct BoxCycle {
+    // state at the cycle's start (as gev_cycle_state)
+    double a, tau, dtau, dtau_old;
+    // static per box
+    double fourpiG, Cf, steplimit, z_relax, pmass;
+    double Om_cdm, Om_b, Om_L, Om_rad, Om_fld, w0, wa;
+    // the cycle's coefficients
+    double a_half;                      // a after the first half RK4 step (drift)
+    double dep_m, dep_m_a;              // deposit mass factors, coeff m / dx^3 (and / a)
+    double prep_bg, prep_c, prep_c2, prep_c3;  // prepareFTsource (scalar), main.cpp:472
+    double pois_c0, pois_c1;            // solveModifiedPoissonFT: coeff / (-N^3), modif
+    double tens_c;                      // prepareFTsource (tensor): 2 fourpiG dx^2 / a
+    double a2dtau;                      // evolveFTvector: a^2 dtau_old
+    double dtk, pk0, pk1, dtd, pd0, pd1;  // kick and drift
+    int32_t nfk, nfd;
+    int32_t on;       // the box takes part in this cycle
+    int32_t on_phi;   // ... and its phi update runs (GR with dtau_old > 0; Newton always)
+    int32_t parity;   // the live particle set: 0 = pos/q/id, 1 = pos2/q2/id2
+    int32_t cycle;
+    int32_t gr;
+    int32_t stop;     // the host ends the box after this cycle's field stage
+};
+
+// A launch over the boxes of a batch: box b's arrays are those of box 0
+// shifted by b * stride bytes.  cyc == nullptr: one box, the kernel's
+// by-value coefficients; otherwise the box index comes from the grid and the
+// coefficients from cyc[box].  flag selects the box's enable flag (0: on,
+// 1: on_phi).
+struct Batch {
+    int64_t stride = 0;
+    const BoxCycle *cyc = nullptr;
+    int flag = 0;
+};
+
+__device__ __forceinline__ bool box_on(const Batch &bt, int b) {
+    return !bt.cyc || (bt.flag ? bt.cyc[b].on_phi : bt.cyc[b].on) != 0;
+}
+template <typename T> __host__ __device__ __forceinline__ T *box_ptr(T *p, int64_t stride, int b) {
+    return p ? reinterpret_cast<T *>(reinterpret_cast<char *>(const_cast<typename std::remove_const<T>::type *>(p)) +
+                                     (int64_t)b * stride)
+             : p;
+}
+
 }  // namespace gev
 
 struct gev_ctx {
+    // device memory: one slab per box (nbox boxes, stride bytes apart); the
+    // pointers below are box 0's.  A view (a box of a batch handed out for the
+    // single-box entry points) has its pointers shifted and owns nothing.
+    void *slab = nullptr;
+    int nbox = 1;
+    int64_t stride = 0;
+    bool view = false;
+
     int N = 0;
     int P = 0;            // k-space row pitch (complex elements)
     int64_t cap = 0;      // particle capacity
@@ -150,18 +233,21 @@ int fft_backward(gev_ctx *ctx, const double2 *kfield, double *real, int ncomp);
 // the x and y passes only (the z pass fused with a solve follows), and the
 // backward transform of a field whose z pass is already done (in place on
 // kfield, which is scratch afterwards)
-int fft_forward_xy(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp);
-int fft_backward_yx(gev_ctx *ctx, double2 *kfield, double *real, int ncomp);
+int fft_forward_xy(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp, const Batch &bt = Batch{});
+int fft_backward_yx(gev_ctx *ctx, double2 *kfield, double *real, int ncomp, const Batch &bt = Batch{});
 // prepareFTsource fused into the first FFT pass (fftsolve.hip): Sij -> SijFT
 // and source (+ phi, chi) -> scalarFT, after the x pass only
-int launch_x_prep_tensor(gev_ctx *c, double coeff);
-int launch_x_prep_scalar(gev_ctx *c, double bg, double coeff, double coeff2, double coeff3);
-int fft_forward_y(gev_ctx *ctx, double2 *kfield, int ncomp);  // the y pass alone, in place
+// A Batch with a BoxCycle array runs every box of the context's batch (box
+// coefficients from the array, the by-value coefficients ignored).
+int launch_x_prep_tensor(gev_ctx *c, double coeff, const Batch &bt = Batch{});
+int launch_x_prep_scalar(gev_ctx *c, double bg, double coeff, double coeff2, double coeff3,
+                         const Batch &bt = Batch{});
+int fft_forward_y(gev_ctx *ctx, double2 *kfield, int ncomp, const Batch &bt = Batch{});  // the y pass alone, in place
 // z pass + solve + inverse z pass (fftsolve.hip); kstore / k0 optional
 int launch_zsolve_poisson(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
-                          double2 *kstore, double2 *k0);
+                          double2 *kstore, double2 *k0, const Batch &bt = Batch{});
 int launch_zsolve_tensor(gev_ctx *c, const double2 *in, double2 *out, double a2dtau, bool evolve,
-                         double2 *kstore, double2 *k0);
+                         double2 *kstore, double2 *k0, const Batch &bt = Batch{});
 int launch_zsolve_vector(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
                          double2 *bift);
 
@@ -205,8 +291,30 @@ int launch_convert(gev_ctx *ctx, const void *src, bool src32, void *dst, bool ds
 
 enum DepositKind { DEP_T00 = 1, DEP_TIJ = 2, DEP_T00_TIJ = 3, DEP_T0I = 4, DEP_CIC = 5 };
 int launch_deposit(gev_ctx *ctx, int kind, double a, int use_phi, double coeff);
+int launch_deposit_batch(gev_ctx *ctx, bool gr, const Batch &bt);
+
+// the main loop's spectrum outputs (api.hip): writeSpectra's delta spectrum
+// and writePowerSpectrum with EXACT_OUTPUT_REDSHIFTS, per box and output
+struct PkWriter {
+    int num_pk = 0;
+    size_t numbins = 0;
+    double rescalek = 0., rescalep = 0.;
+    std::vector<double> kb, pw, ks, ps, p2, pre, pre_z;
+    std::vector<int32_t> oc, have;
+    void init(const gev_ctx *c, const gev_sim_params *p, int num_pk, int numbins, double boxsize);
+};
+// output j at z_pk = zt from the box's current state into table j of the
+// [num_pk][numbins] outputs
+int write_spectrum(gev_ctx *c, const gev_sim_params *p, const gev_cycle_state *st, PkWriter &w, int j,
+                   double zt, double *k, double *pk, double *sk, double *spk, int32_t *occ,
+                   gev_pk_output_info *info);
+// batches (api.hip): a box's array pointers at its slab; a batch's contexts
+void assign_box_pointers(gev_ctx *c, char *base);
+int create_ctx(const gev_box_desc *d, int nbox, gev_ctx **out);
+int64_t slab_bytes(int N, int64_t cap, bool fp32);
 
 int launch_refile(gev_ctx *ctx);  // cells from positions, then CSR
+int launch_particles_batch(gev_ctx *ctx, bool gr, const Batch &bt);
 int launch_particle_update(gev_ctx *ctx, bool kick, int qm, double dtk, int nfk, const double pk[2],
                            bool drift, int pm, double dtd, int nfd, const double pd[2],
                            double *maxvel);

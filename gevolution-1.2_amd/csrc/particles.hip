@@ -87,7 +87,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
 }
 
 __global__ __launch_bounds__(TPB) void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n,
-                                                     uint32_t *__restrict__ part) {
+                                                     uint32_t *__restrict__ part, gev::Batch bt) {
+    if (!gev::box_on(bt, blockIdx.y)) return;
+    in = gev::box_ptr(in, bt.stride, blockIdx.y);
+    part = gev::box_ptr(part, bt.stride, blockIdx.y);
     const int64_t b0 = (int64_t)blockIdx.x * SCAN_BLOCK;
     uint4 v[SCAN_PASSES];
 #pragma unroll
@@ -109,7 +112,9 @@ __global__ __launch_bounds__(TPB) void k_scan_reduce(const uint32_t *__restrict_
 }
 
 // exclusive scan of the block partials in place (single workgroup)
-__global__ __launch_bounds__(TPB) void k_scan_partials(uint32_t *__restrict__ part, int64_t nblk) {
+__global__ __launch_bounds__(TPB) void k_scan_partials(uint32_t *__restrict__ part, int64_t nblk, gev::Batch bt) {
+    if (!gev::box_on(bt, blockIdx.y)) return;
+    part = gev::box_ptr(part, bt.stride, blockIdx.y);
     const int64_t per = (nblk + TPB - 1) / TPB;
     const int64_t b = (int64_t)threadIdx.x * per;
     uint32_t s = 0;
@@ -127,7 +132,11 @@ __global__ __launch_bounds__(TPB) void k_scan_partials(uint32_t *__restrict__ pa
 
 __global__ __launch_bounds__(TPB) void k_scan_final(const uint32_t *__restrict__ cnt, int64_t n,
                                                     const uint32_t *__restrict__ part,
-                                                    uint32_t *__restrict__ out) {
+                                                    uint32_t *__restrict__ out, gev::Batch bt) {
+    if (!gev::box_on(bt, blockIdx.y)) return;
+    cnt = gev::box_ptr(cnt, bt.stride, blockIdx.y);
+    part = gev::box_ptr(part, bt.stride, blockIdx.y);
+    out = gev::box_ptr(out, bt.stride, blockIdx.y);
     const int64_t b0 = (int64_t)blockIdx.x * SCAN_BLOCK;
     uint4 v[SCAN_PASSES];
 #pragma unroll
@@ -188,7 +197,12 @@ __device__ __forceinline__ CellRun cell_run(uint32_t c) {
 // One atomic per run of equal cells in a wavefront.
 __global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ cell, const uint32_t *__restrict__ start,
                                                  uint32_t *__restrict__ count, int64_t n,
-                                                 uint32_t *__restrict__ perm) {
+                                                 uint32_t *__restrict__ perm, gev::Batch bt) {
+    if (!gev::box_on(bt, blockIdx.y)) return;
+    cell = gev::box_ptr(cell, bt.stride, blockIdx.y);
+    start = gev::box_ptr(start, bt.stride, blockIdx.y);
+    count = gev::box_ptr(count, bt.stride, blockIdx.y);
+    perm = gev::box_ptr(perm, bt.stride, blockIdx.y);
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= n) return;
     const uint32_t c = cell[p];
@@ -243,9 +257,23 @@ __device__ __forceinline__ void cswap(uint32_t *v, int64_t *k, int a, int b) {
 // thread per cell: the tiny sorts; a segment with a larger cell is flagged
 // (a flag per segment: one shared list counter would serialise the late,
 // clustered cycles, which flag most segments, on one atomic)
+// the ID array the filing sorts by: in a batch, the set the drift just wrote
+// (the other one than the box's live set), at the box's slab
+__device__ __forceinline__ const int64_t *sort_ids(const int64_t *id, const int64_t *id_alt, const gev::Batch &bt,
+                                                   int b) {
+    if (!bt.cyc) return id;
+    return gev::box_ptr(bt.cyc[b].parity ? id : id_alt, bt.stride, b);
+}
+
 __global__ __launch_bounds__(TPB) void k_sort_cells(const uint32_t *__restrict__ start,
                                                     const int64_t *__restrict__ id, int64_t n3v,
-                                                    uint32_t *__restrict__ perm, uint8_t *__restrict__ flag) {
+                                                    uint32_t *__restrict__ perm, uint8_t *__restrict__ flag,
+                                                    const int64_t *__restrict__ id_alt, gev::Batch bt) {
+    if (!gev::box_on(bt, blockIdx.y)) return;
+    id = sort_ids(id, id_alt, bt, blockIdx.y);
+    start = gev::box_ptr(start, bt.stride, blockIdx.y);
+    perm = gev::box_ptr(perm, bt.stride, blockIdx.y);
+    flag = gev::box_ptr(flag, bt.stride, blockIdx.y);
     const int64_t c = (int64_t)blockIdx.x * TPB + threadIdx.x;
     const uint32_t b0 = c < n3v ? start[c] : 0u;
     const uint32_t m = c < n3v ? start[c + 1] - b0 : 0u;
@@ -283,7 +311,15 @@ __global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restric
                                                        const int64_t *__restrict__ id, int64_t n3v,
                                                        uint32_t *__restrict__ perm, const uint8_t *__restrict__ flag,
                                                        uint32_t nsegs, uint32_t *__restrict__ big,
-                                                       uint32_t *__restrict__ nbig) {
+                                                       uint32_t *__restrict__ nbig, const int64_t *__restrict__ id_alt,
+                                                       gev::Batch bt) {
+    if (!gev::box_on(bt, blockIdx.y)) return;
+    id = sort_ids(id, id_alt, bt, blockIdx.y);
+    start = gev::box_ptr(start, bt.stride, blockIdx.y);
+    perm = gev::box_ptr(perm, bt.stride, blockIdx.y);
+    flag = gev::box_ptr(flag, bt.stride, blockIdx.y);
+    big = gev::box_ptr(big, bt.stride, blockIdx.y);
+    nbig = gev::box_ptr(nbig, bt.stride, blockIdx.y);
     __shared__ int64_t s_key[SORT_SEG];
     __shared__ uint32_t s_val[SORT_SEG];
     __shared__ uint32_t s_out[SORT_SEG];
@@ -357,8 +393,15 @@ __global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restric
 // store each.  Cells above SORT_LDS particles fall back to one thread.
 __global__ __launch_bounds__(TPB) void k_sort_big(const uint32_t *__restrict__ start, const int64_t *__restrict__ id,
                                                   uint32_t *__restrict__ perm, const uint32_t *__restrict__ big,
-                                                  const uint32_t *__restrict__ nbig) {
+                                                  const uint32_t *__restrict__ nbig, const int64_t *__restrict__ id_alt,
+                                                  gev::Batch bt) {
     // (nbig: the big list's count)
+    if (!gev::box_on(bt, blockIdx.y)) return;
+    id = sort_ids(id, id_alt, bt, blockIdx.y);
+    start = gev::box_ptr(start, bt.stride, blockIdx.y);
+    perm = gev::box_ptr(perm, bt.stride, blockIdx.y);
+    big = gev::box_ptr(big, bt.stride, blockIdx.y);
+    nbig = gev::box_ptr(nbig, bt.stride, blockIdx.y);
     __shared__ int64_t s_key[SORT_LDS];
     __shared__ uint32_t s_val[SORT_LDS];
     const uint32_t nb = *nbig;
@@ -800,14 +843,55 @@ struct StepArgs {
 #define GEV_KICK_OCC
 #endif
 
-template <bool KICK, int QM, bool DRIFT, int PM, bool POW2, typename F>
-__global__ __launch_bounds__(TPB, (sizeof(F) == 4 ? GEV_KICK_WAVES_F32 : 1)) GEV_KICK_OCC void k_update_tiled(
+// BATCH: the launch covers the boxes of a batch (its own instantiation, so
+// the one-box kernel keeps its register budget).
+template <bool KICK, int QM, bool DRIFT, int PM, bool POW2, typename F, bool BATCH = false>
+__global__ __launch_bounds__(TPB, (sizeof(F) == 4 && !BATCH ? GEV_KICK_WAVES_F32 : 1)) GEV_KICK_OCC void k_update_tiled(
     double *__restrict__ pos, double *__restrict__ q, int64_t cap, int N, int ntx, int nty, int ntz,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const F *__restrict__ phi,
     const F *__restrict__ chi, const F *__restrict__ Bi, StepArgs sa,
     unsigned long long *__restrict__ vmax, uint32_t *__restrict__ count, uint32_t *__restrict__ cell,
     double *__restrict__ pos_out, double *__restrict__ q_out,
-    const int64_t *__restrict__ id_in, int64_t *__restrict__ id_out) {
+    const int64_t *__restrict__ id_in, int64_t *__restrict__ id_out, gev::Batch bt) {
+    if constexpr (BATCH) {
+        // box blockIdx.y of a batch: its arrays, the live set in and the other out, its kick and drift
+        const int b = blockIdx.y;
+        if (!gev::box_on(bt, b)) return;
+        const gev::BoxCycle &bc = bt.cyc[b];
+        if (bc.parity) {
+            double *t = pos;
+            pos = pos_out;
+            pos_out = t;
+            t = q;
+            q = q_out;
+            q_out = t;
+            const int64_t *u = id_in;
+            id_in = id_out;
+            id_out = const_cast<int64_t *>(u);
+        }
+        pos = gev::box_ptr(pos, bt.stride, b);
+        q = gev::box_ptr(q, bt.stride, b);
+        pos_out = gev::box_ptr(pos_out, bt.stride, b);
+        q_out = gev::box_ptr(q_out, bt.stride, b);
+        id_in = gev::box_ptr(id_in, bt.stride, b);
+        id_out = gev::box_ptr(id_out, bt.stride, b);
+        start = gev::box_ptr(start, bt.stride, b);
+        perm = gev::box_ptr(perm, bt.stride, b);
+        phi = gev::box_ptr(phi, bt.stride, b);
+        chi = gev::box_ptr(chi, bt.stride, b);
+        Bi = gev::box_ptr(Bi, bt.stride, b);
+        vmax = gev::box_ptr(vmax, bt.stride, b);
+        count = gev::box_ptr(count, bt.stride, b);
+        cell = gev::box_ptr(cell, bt.stride, b);
+        sa.nfk = bc.nfk;
+        sa.nfd = bc.nfd;
+        sa.dtk = bc.dtk;
+        sa.dtd = bc.dtd;
+        sa.pk[0] = bc.pk0;
+        sa.pk[1] = bc.pk1;
+        sa.pd[0] = bc.pd0;
+        sa.pd[1] = bc.pd1;
+    }
     __shared__ F s_phi[Region<-1>::SIZE], s_chi[Region<-1>::SIZE];
     __shared__ F s_b0[Region<0>::SIZE], s_b1[Region<1>::SIZE], s_b2[Region<2>::SIZE];
     __shared__ unsigned long long sh[TPB / 64];
@@ -1030,71 +1114,86 @@ __global__ __launch_bounds__(TPB, (sizeof(F) == 4 ? GEV_KICK_WAVES_F32 : 1)) GEV
 }
 
 template <bool KICK, int QM, bool DRIFT, int PM, typename F>
-void launch_update_f(gev_ctx *ctx, const StepArgs &sa) {
+void launch_update_f(gev_ctx *ctx, const StepArgs &sa, const gev::Batch &bt) {
     const int N = ctx->N;
     const int ntx = (N + KTX - 1) / KTX, nty = (N + KTY - 1) / KTY, ntz = (N + KTZ - 1) / KTZ;
     const int64_t nblk = (int64_t)ntx * nty * ntz;
+    const unsigned nbox = bt.cyc ? (unsigned)ctx->nbox : 1u;
     const F *phi = gev::fld<F>(ctx->phi), *chi = gev::fld<F>(ctx->chi), *Bi = gev::fld<F>(ctx->Bi);
-    if ((N & (N - 1)) == 0)
-        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, true, F>), dim3((unsigned)nblk), dim3(TPB), 0,
-                           ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           phi, chi, Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->pos2, ctx->q2, ctx->id,
-                           ctx->id2);
-    else
-        hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, false, F>), dim3((unsigned)nblk), dim3(TPB), 0,
-                           ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm,
-                           phi, chi, Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->pos2, ctx->q2, ctx->id,
-                           ctx->id2);
+#define GEV_KICK_LAUNCH(P2, B)                                                                                 \
+    hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, P2, F, B>), dim3((unsigned)nblk, nbox), dim3(TPB), 0, \
+                       ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm, phi, chi,  \
+                       Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->pos2, ctx->q2, ctx->id, ctx->id2, bt)
+    const bool pow2 = (N & (N - 1)) == 0;
+    if (bt.cyc) {
+        // the batched cycle runs the kick and drift together only
+        if constexpr (KICK && DRIFT) {
+            if (pow2) GEV_KICK_LAUNCH(true, true);
+            else GEV_KICK_LAUNCH(false, true);
+        }
+    } else if (pow2) {
+        GEV_KICK_LAUNCH(true, false);
+    } else {
+        GEV_KICK_LAUNCH(false, false);
+    }
+#undef GEV_KICK_LAUNCH
 }
 
 template <bool KICK, int QM, bool DRIFT, int PM>
-void launch_update_t(gev_ctx *ctx, const StepArgs &sa) {
-    if (ctx->fp32) launch_update_f<KICK, QM, DRIFT, PM, float>(ctx, sa);
-    else launch_update_f<KICK, QM, DRIFT, PM, double>(ctx, sa);
+void launch_update_t(gev_ctx *ctx, const StepArgs &sa, const gev::Batch &bt = gev::Batch{}) {
+    if (ctx->fp32) launch_update_f<KICK, QM, DRIFT, PM, float>(ctx, sa, bt);
+    else launch_update_f<KICK, QM, DRIFT, PM, double>(ctx, sa, bt);
 }
 
 }  // namespace
 
 namespace gev {
 
-// scan count[] into start[] and build perm[] from cell[] (count[] ends zero)
-static int finish_filing(gev_ctx *ctx) {
+// scan count[] into start[] and build perm[] from cell[] (count[] ends zero).
+// Over a batch (bt.cyc set) every box is filed by the same launches; the
+// list counter sort_n of each box was cleared before the particle stage.
+static int finish_filing(gev_ctx *ctx, const Batch &bt = Batch{}) {
     const int64_t n3v = n3(ctx);
     const int64_t nblk = (n3v + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const unsigned nb = bt.cyc ? (unsigned)ctx->nbox : 1u;
     {
         ScopedTimer t(ctx, "filing_scan");
-        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblk), dim3(TPB), 0, ctx->stream, ctx->count,
-                           n3v, ctx->scan_part);
-        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(TPB), 0, ctx->stream, ctx->scan_part, nblk);
-        hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nblk), dim3(TPB), 0, ctx->stream, ctx->count,
-                           n3v, ctx->scan_part, ctx->start);
+        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblk, nb), dim3(TPB), 0, ctx->stream, ctx->count,
+                           n3v, ctx->scan_part, bt);
+        hipLaunchKernelGGL(k_scan_partials, dim3(1, nb), dim3(TPB), 0, ctx->stream, ctx->scan_part, nblk, bt);
+        hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nblk, nb), dim3(TPB), 0, ctx->stream, ctx->count,
+                           n3v, ctx->scan_part, ctx->start, bt);
     }
     if (ctx->n > 0) {
         ScopedTimer t(ctx, "filing_scatter");
-        hipLaunchKernelGGL(k_scatter, dim3(grid_for(ctx->n, TPB)), dim3(TPB), 0, ctx->stream,
-                           ctx->cell, ctx->start, ctx->count, ctx->n, ctx->perm);
+        hipLaunchKernelGGL(k_scatter, dim3(grid_for(ctx->n, TPB), nb), dim3(TPB), 0, ctx->stream,
+                           ctx->cell, ctx->start, ctx->count, ctx->n, ctx->perm, bt);
     }
     {
         ScopedTimer t(ctx, "filing_sort");
         // sort_n[1]: the number of listed cells
-        int rc = check_hip(ctx, hipMemsetAsync(ctx->sort_n, 0, 2 * sizeof(uint32_t), ctx->stream), "sort list clear");
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(n3v, TPB)), dim3(TPB), 0, ctx->stream,
-                           ctx->start, ctx->id, n3v, ctx->perm, reinterpret_cast<uint8_t *>(ctx->sort_segs));
+        if (!bt.cyc) {
+            int rc = check_hip(ctx, hipMemsetAsync(ctx->sort_n, 0, 2 * sizeof(uint32_t), ctx->stream),
+                               "sort list clear");
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(n3v, TPB), nb), dim3(TPB), 0, ctx->stream,
+                           ctx->start, ctx->id, n3v, ctx->perm, reinterpret_cast<uint8_t *>(ctx->sort_segs), ctx->id2,
+                           bt);
     }
     {
         ScopedTimer t(ctx, "filing_sort_segments");
         // SORT_SEG_BATCH segments per workgroup: the late, clustered cycles flag
         // most segments and need the parallelism; early ones flag almost none
         const unsigned nsw = (unsigned)((grid_for(n3v, TPB) + SORT_SEG_BATCH - 1) / SORT_SEG_BATCH);
-        hipLaunchKernelGGL(k_sort_segments, dim3(nsw), dim3(TPB), 0, ctx->stream, ctx->start, ctx->id, n3v,
+        hipLaunchKernelGGL(k_sort_segments, dim3(nsw, nb), dim3(TPB), 0, ctx->stream, ctx->start, ctx->id, n3v,
                            ctx->perm, reinterpret_cast<const uint8_t *>(ctx->sort_segs),
-                           (uint32_t)grid_for(n3v, TPB), ctx->sort_big, ctx->sort_n + 1);
+                           (uint32_t)grid_for(n3v, TPB), ctx->sort_big, ctx->sort_n + 1, ctx->id2, bt);
     }
     {
         ScopedTimer t(ctx, "filing_sort_dense_cells");
-        hipLaunchKernelGGL(k_sort_big, dim3(1024), dim3(TPB), 0, ctx->stream, ctx->start, ctx->id, ctx->perm,
-                           ctx->sort_big, ctx->sort_n + 1);
+        hipLaunchKernelGGL(k_sort_big, dim3(1024, nb), dim3(TPB), 0, ctx->stream, ctx->start, ctx->id, ctx->perm,
+                           ctx->sort_big, ctx->sort_n + 1, ctx->id2, bt);
     }
     const int rc = check_hip(ctx, hipGetLastError(), "filing launch");
     if (!rc) ctx->count_clean = true;  // the scatter counted every cell back to zero
@@ -1114,6 +1213,22 @@ int launch_refile(gev_ctx *ctx) {
     rc = check_hip(ctx, hipGetLastError(), "count launch");
     if (rc) return rc;
     return finish_filing(ctx);
+}
+
+// The particle stage over the boxes of a batch: kick + drift of every box
+// (its live set in, the other set out, its coefficients from its BoxCycle),
+// then the filing.  No clearing here: the batch clears vmax and the sort
+// list counter of every box before this stage, and every completed filing
+// leaves the counts at zero.
+int launch_particles_batch(gev_ctx *ctx, bool gr, const Batch &bt) {
+    StepArgs sa{};
+    if (ctx->n > 0) {
+        ScopedTimer t(ctx, "kick_drift");
+        if (gr) launch_update_t<true, 0, true, 0>(ctx, sa, bt);
+        else launch_update_t<true, 1, true, 1>(ctx, sa, bt);
+    }
+    int rc = check_hip(ctx, hipGetLastError(), "particle update launch");
+    return rc ? rc : finish_filing(ctx, bt);
 }
 
 int launch_particle_update(gev_ctx *ctx, bool kick, int qm, double dtk, int nfk, const double pk[2],

@@ -92,8 +92,8 @@ __global__ __launch_bounds__(TPB) void k_sum_final(const double *__restrict__ pa
 }
 
 __global__ __launch_bounds__(TPB) void k_fill_zero(double2 *__restrict__ p, int64_t n2) {
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i < n2) p[i] = make_double2(0., 0.);
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n2; i += (int64_t)gridDim.x * TPB)
+        p[i] = make_double2(0., 0.);
 }
 
 // element-wise copy between the two forms' field elements
@@ -166,8 +166,10 @@ int launch_fill_zero(gev_ctx *ctx, void *ptr, size_t bytes) {
     if (bytes % 16 != 0 || bytes == 0) return fail(ctx, GEV_EINVAL, "fill size must be a positive multiple of 16");
     const int64_t n2 = (int64_t)(bytes / 16);
     ScopedTimer t(ctx, "projection_init");
-    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n2, TPB)), dim3(TPB), 0, ctx->stream,
-                       (double2 *)ptr, n2);
+    // grid-stride: a batch's slabs can exceed the 2^32 work-items of one launch dimension
+    const int64_t blocks = (n2 + TPB - 1) / TPB;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)(blocks < (1 << 20) ? blocks : (1 << 20))), dim3(TPB), 0,
+                       ctx->stream, (double2 *)ptr, n2);
     return check_hip(ctx, hipGetLastError(), "fill launch");
 }
 

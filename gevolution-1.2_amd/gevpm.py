@@ -48,6 +48,9 @@ EXPORTED = [
     "gev_generate_ic_basic", "gev_extract_cross_spectrum", "gev_extract_power_spectrum",
     "gev_write_power_spectrum", "gev_hibernate", "gev_restart", "gev_run_cycles_recorded",
     "gev_write_background", "gev_copy_state", "gev_run_to_outputs", "gev_extract_power_spectrum_noalloc",
+    "gev_batch_footprint_bytes", "gev_batch_create", "gev_batch_destroy", "gev_batch_last_error", "gev_batch_box",
+    "gev_batch_start_box", "gev_batch_stop_box", "gev_batch_get_state", "gev_batch_run_cycles",
+    "gev_batch_set_outputs", "gev_batch_step_outputs", "gev_batch_run_to_outputs", "gev_batch_get_outputs",
 ]
 
 
@@ -79,6 +82,11 @@ class ICParams(ctypes.Structure):
 class BackgroundRecord(ctypes.Structure):
     _fields_ = [("cycle", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("tau", _d), ("a", _d),
                 ("hconf_ratio", _d), ("phi_k0", _d), ("T00hom", _d)]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("ngrid", ctypes.c_int32), ("form", ctypes.c_int32), ("nbox", ctypes.c_int32),
+                ("gr_flag", ctypes.c_int32), ("npart", ctypes.c_int64), ("stream", ctypes.c_void_p)]
 
 
 class PkOutputInfo(ctypes.Structure):
@@ -169,6 +177,22 @@ def lib():
                                                ctypes.c_char_p, _d, _d]
         L.gev_extract_power_spectrum_noalloc.argtypes = [_p, _i, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                          _p, _p, _p, _p, _p]
+        _i32 = ctypes.c_int32
+        L.gev_batch_footprint_bytes.restype = _i64
+        L.gev_batch_footprint_bytes.argtypes = [_p]
+        L.gev_batch_create.argtypes = [_p, ctypes.POINTER(_p)]
+        L.gev_batch_destroy.argtypes = [_p]
+        L.gev_batch_last_error.restype = ctypes.c_char_p
+        L.gev_batch_last_error.argtypes = [_p]
+        L.gev_batch_box.argtypes = [_p, _i32, ctypes.POINTER(_p)]
+        L.gev_batch_start_box.argtypes = [_p, _i32, _p, _p, _d]
+        L.gev_batch_stop_box.argtypes = [_p, _i32]
+        L.gev_batch_get_state.argtypes = [_p, _i32, _p]
+        L.gev_batch_run_cycles.argtypes = [_p, _i32, _i32]
+        L.gev_batch_set_outputs.argtypes = [_p, _i32, _p, _i32]
+        L.gev_batch_step_outputs.argtypes = [_p, _i32, _p]
+        L.gev_batch_run_to_outputs.argtypes = [_p, _i32, _i32]
+        L.gev_batch_get_outputs.argtypes = [_p, _i32, _p, _p, _p, _p, _p, _p]
         L.gev_run_to_outputs.argtypes = [_p, _p, _p, ctypes.c_int32, _p, ctypes.c_int32, _d, ctypes.c_int32,
                                          _p, _p, _p, _p, _p, _p]
         _lib = L
@@ -210,7 +234,17 @@ def footprint_bytes(ngrid, npart, form=FORM_FP64):
 class Box:
     """One simulation box resident on the current GPU (a gev_ctx)."""
 
+    @classmethod
+    def _view(cls, handle, ngrid, npart, form):
+        """A box of a batch (owned by the batch: close() leaves it alone)."""
+        b = cls.__new__(cls)
+        b.N, b.cap, b.form = int(ngrid), int(npart), int(form)
+        b._c = handle
+        b._owned = False
+        return b
+
     def __init__(self, ngrid, npart, stream=None, form=FORM_FP64):
+        self._owned = True
         self.N = int(ngrid)
         self.cap = int(npart)
         self.form = int(form)
@@ -227,9 +261,9 @@ class Box:
             raise GevError("%s failed (%d): %s" % (what, rc, lib().gev_last_error(self._c).decode()))
 
     def close(self):
-        if self._c:
+        if self._c and getattr(self, "_owned", True):
             lib().gev_ctx_destroy(self._c)
-            self._c = _p()
+        self._c = _p()
 
     def __del__(self):
         try:
@@ -496,6 +530,94 @@ class Box:
         for line in buf.value.decode().splitlines():
             name, calls, ms = line.split()
             out[name] = (int(calls), float(ms))
+        return out
+
+
+def batch_footprint_bytes(ngrid, npart, nbox, form=FORM_FP64, gr=1):
+    d = BatchDesc(ngrid, form, nbox, gr, npart, None)
+    return lib().gev_batch_footprint_bytes(ctypes.byref(d))
+
+
+class Batch:
+    """nbox boxes of one shape stepped by the same launches (gev_batch)."""
+
+    def __init__(self, ngrid, npart, nbox, gr=1, form=FORM_FP64, stream=None):
+        self.N, self.cap, self.nbox, self.gr, self.form = int(ngrid), int(npart), int(nbox), int(gr), int(form)
+        self._b = _p()
+        d = BatchDesc(self.N, self.form, self.nbox, self.gr, self.cap, stream)
+        rc = lib().gev_batch_create(ctypes.byref(d), ctypes.byref(self._b))
+        if rc != 0:
+            raise GevError("gev_batch_create(N=%d, npart=%d, nbox=%d) failed with %d" % (self.N, self.cap, self.nbox, rc))
+        self.numbins = 0
+        self.num_pk = 0
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise GevError("%s failed (%d): %s" % (what, rc, lib().gev_batch_last_error(self._b).decode()))
+
+    def close(self):
+        if self._b:
+            lib().gev_batch_destroy(self._b)
+            self._b = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def box(self, b):
+        """Box b as a Box (for IC generation, uploads, downloads, spectra)."""
+        h = _p()
+        self._chk(lib().gev_batch_box(self._b, b, ctypes.byref(h)), "batch_box")
+        return Box._view(h, self.N, self.cap, self.form)
+
+    def start_box(self, b, sim, state, boxsize):
+        self._chk(lib().gev_batch_start_box(self._b, b, ctypes.byref(sim), ctypes.byref(state), float(boxsize)),
+                  "batch_start_box")
+
+    def stop_box(self, b):
+        self._chk(lib().gev_batch_stop_box(self._b, b), "batch_stop_box")
+
+    def state(self, b):
+        st = CycleState()
+        self._chk(lib().gev_batch_get_state(self._b, b, ctypes.byref(st)), "batch_get_state")
+        return st
+
+    def run_cycles(self, ncycles, graph=True):
+        self._chk(lib().gev_batch_run_cycles(self._b, ncycles, int(graph)), "batch_run_cycles")
+
+    def set_outputs(self, z_pk, numbins):
+        z = np.ascontiguousarray(z_pk, dtype=np.float64)
+        self._chk(lib().gev_batch_set_outputs(self._b, len(z), _ptr(z), numbins), "batch_set_outputs")
+        self.num_pk, self.numbins = len(z), int(numbins)
+
+    def step(self, graph=True):
+        """One cycle with outputs; the boxes that finished in it."""
+        fin = np.zeros(self.nbox, dtype=np.int32)
+        self._chk(lib().gev_batch_step_outputs(self._b, int(graph), _ptr(fin)), "batch_step_outputs")
+        return np.nonzero(fin)[0].tolist()
+
+    def run_to_outputs(self, graph=True, max_cycles=100000):
+        self._chk(lib().gev_batch_run_to_outputs(self._b, int(graph), max_cycles), "batch_run_to_outputs")
+
+    def outputs(self, b):
+        m, nb = self.num_pk, self.numbins
+        out = {n: np.zeros((m, nb)) for n in ("k", "pk", "sigma_k", "sigma_pk")}
+        out["occupation"] = np.zeros((m, nb), dtype=np.int32)
+        info = (PkOutputInfo * m)()
+        self._chk(lib().gev_batch_get_outputs(self._b, b, *[_ptr(out[n]) for n in
+                                              ("k", "pk", "sigma_k", "sigma_pk", "occupation")], info),
+                  "batch_get_outputs")
+        out["z"] = np.array([i.z for i in info])
+        out["cycle"] = np.array([i.cycle for i in info], dtype=np.int32)
+        out["interpolated"] = np.array([i.interpolated for i in info], dtype=bool)
         return out
 
 

@@ -416,6 +416,62 @@ int gev_run_cycles_recorded(gev_ctx *ctx, const gev_sim_params *p, gev_cycle_sta
    (main.cpp:531-533: header before the cycle-0 line).  No context. */
 int gev_write_background(const char *path, const gev_background_record *rec, int n);
 
+/* ---- batches: many boxes stepped by the same launches -------------------- */
+
+/* No counterpart in the reference, which runs one simulation per process
+   (main.cpp:115-149): a batch holds nbox boxes of the same grid, particle
+   count, form and gravity (GR or Newtonian; parabolic B) and runs the main
+   loop of main.cpp:372-879 for all of them with one launch per kernel, each
+   box with its own cosmology, seed, time step and outputs.  The per-cycle
+   host arithmetic of gev_run_cycles (the background step and every call's
+   coefficients) runs on the device with the same expressions, so every
+   box's results equal those of the same box run alone with gev_run_cycles /
+   gev_run_to_outputs bit for bit, and each stage of the cycle can be
+   captured once in a HIP graph and replayed (use_graph). */
+typedef struct gev_batch gev_batch;
+
+typedef struct {
+    int32_t ngrid;     /* N of every box                                */
+    int32_t form;      /* GEV_FORM_FP64 or GEV_FORM_FP32                 */
+    int32_t nbox;      /* boxes                                          */
+    int32_t gr_flag;   /* 1 = GR, 0 = Newton, for every box              */
+    int64_t npart;     /* particles of every box                         */
+    void *stream;      /* hipStream_t; NULL = a new stream               */
+} gev_batch_desc;
+
+int64_t gev_batch_footprint_bytes(const gev_batch_desc *desc);
+int gev_batch_create(const gev_batch_desc *desc, gev_batch **out);
+int gev_batch_destroy(gev_batch *batch);
+const char *gev_batch_last_error(const gev_batch *batch);
+/* The single-box context of box b: every entry point above works on it (IC
+   generation, uploads, downloads, spectra), on the batch's stream, between
+   batch calls; the batch's next cycle starts from whatever it then holds.
+   Owned by the batch (gev_ctx_destroy refuses it). */
+int gev_batch_box(gev_batch *batch, int32_t b, gev_ctx **view);
+/* Box b takes part in the batch's cycles from state *st (stage 0) with its
+   own parameters (p->gr_flag equal to the batch's, p->vector_flag 0); its
+   particles (npart of them, filed) and fields are what its view holds;
+   boxsize in Mpc/h scales its spectrum outputs. */
+int gev_batch_start_box(gev_batch *batch, int32_t b, const gev_sim_params *p, const gev_cycle_state *st,
+                        double boxsize);
+int gev_batch_stop_box(gev_batch *batch, int32_t b);
+/* Box b's state as the batch advanced it (maxvel read at the end of runs). */
+int gev_batch_get_state(gev_batch *batch, int32_t b, gev_cycle_state *st);
+/* ncycles cycles of every running box (no outputs), as gev_run_cycles. */
+int gev_batch_run_cycles(gev_batch *batch, int32_t ncycles, int32_t use_graph);
+/* The delta power-spectrum outputs of every box, as gev_run_to_outputs
+   writes them: at z_pk[0..num_pk) (strictly descending), numbins bins. */
+int gev_batch_set_outputs(gev_batch *batch, int32_t num_pk, const double *z_pk, int32_t numbins);
+/* One cycle of every running box with its outputs; a box that writes its
+   last output stops after that cycle's field stage (finished[b] = 1 for the
+   boxes that finished in this call; finished may be NULL). */
+int gev_batch_step_outputs(gev_batch *batch, int32_t use_graph, int32_t *finished);
+/* Steps until every running box has written its last output. */
+int gev_batch_run_to_outputs(gev_batch *batch, int32_t use_graph, int32_t max_cycles);
+/* The [num_pk][numbins] tables of a finished box (see gev_run_to_outputs). */
+int gev_batch_get_outputs(gev_batch *batch, int32_t b, double *k, double *pk, double *sigma_k, double *sigma_pk,
+                          int32_t *occupation, gev_pk_output_info *info);
+
 /* ---- timing (the -DBENCHMARK categories of main.cpp:71-88) --------------- */
 
 /* When enabled, every kernel launch is bracketed by HIP events on the context
