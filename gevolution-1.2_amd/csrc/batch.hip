@@ -395,6 +395,12 @@ int batch_cycle(gev_batch *B, bool use_graph, bool outputs) {
 int verify_scalars(gev_batch *B) {
     gev_ctx *c = B->base;
     std::vector<BoxCycle> d(c->nbox);
+    sync_views(B);
+    if (B->dirty) {
+        // boxes started since the last cycle: the device takes their state first
+        int ru = upload_scalars(B);
+        if (ru) return ru;
+    }
     int rc = batch_hip(B, hipMemcpyAsync(d.data(), B->cyc_d, sizeof(BoxCycle) * c->nbox, hipMemcpyDeviceToHost,
                                          c->stream), "batch scalars readback");
     if (!rc) rc = batch_hip(B, hipStreamSynchronize(c->stream), "batch sync");
@@ -602,6 +608,12 @@ GEV_EXPORT int gev_batch_run_to_outputs(gev_batch *B, int32_t use_graph, int32_t
         int rc = gev_batch_step_outputs(B, use_graph, nullptr);
         if (rc) return rc;
     }
+    int rc = verify_scalars(B);
+    return rc ? rc : read_maxvel(B);
+}
+
+GEV_EXPORT int gev_batch_synchronize(gev_batch *B) {
+    if (!B) return GEV_EINVAL;
     int rc = verify_scalars(B);
     return rc ? rc : read_maxvel(B);
 }

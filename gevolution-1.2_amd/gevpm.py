@@ -51,6 +51,7 @@ EXPORTED = [
     "gev_batch_footprint_bytes", "gev_batch_create", "gev_batch_destroy", "gev_batch_last_error", "gev_batch_box",
     "gev_batch_start_box", "gev_batch_stop_box", "gev_batch_get_state", "gev_batch_run_cycles",
     "gev_batch_set_outputs", "gev_batch_step_outputs", "gev_batch_run_to_outputs", "gev_batch_get_outputs",
+    "gev_batch_synchronize",
 ]
 
 
@@ -193,6 +194,7 @@ def lib():
         L.gev_batch_step_outputs.argtypes = [_p, _i32, _p]
         L.gev_batch_run_to_outputs.argtypes = [_p, _i32, _i32]
         L.gev_batch_get_outputs.argtypes = [_p, _i32, _p, _p, _p, _p, _p, _p]
+        L.gev_batch_synchronize.argtypes = [_p]
         L.gev_run_to_outputs.argtypes = [_p, _p, _p, ctypes.c_int32, _p, ctypes.c_int32, _d, ctypes.c_int32,
                                          _p, _p, _p, _p, _p, _p]
         _lib = L
@@ -606,6 +608,10 @@ class Batch:
 
     def run_to_outputs(self, graph=True, max_cycles=100000):
         self._chk(lib().gev_batch_run_to_outputs(self._b, int(graph), max_cycles), "batch_run_to_outputs")
+
+    def synchronize(self):
+        """Waits for the batch; checks the device time stepping against the host's."""
+        self._chk(lib().gev_batch_synchronize(self._b), "batch_synchronize")
 
     def outputs(self, b):
         m, nb = self.num_pk, self.numbins

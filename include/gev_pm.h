@@ -466,6 +466,9 @@ int gev_batch_set_outputs(gev_batch *batch, int32_t num_pk, const double *z_pk, 
    last output stops after that cycle's field stage (finished[b] = 1 for the
    boxes that finished in this call; finished may be NULL). */
 int gev_batch_step_outputs(gev_batch *batch, int32_t use_graph, int32_t *finished);
+/* Waits for the batch's work; checks every running box's device time
+   stepping (a, tau, dtau, cycle) against the host's, and reads maxvel. */
+int gev_batch_synchronize(gev_batch *batch);
 /* Steps until every running box has written its last output. */
 int gev_batch_run_to_outputs(gev_batch *batch, int32_t use_graph, int32_t max_cycles);
 /* The [num_pk][numbins] tables of a finished box (see gev_run_to_outputs). */

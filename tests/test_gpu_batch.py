@@ -142,3 +142,23 @@ def test_batch_box_against_the_oracle(gev):
     err = np.max(np.abs(got["pk"][0] - want["pk"][0])[live] / want["pk"][0][live])
     print("batch box vs oracle: %d cycles, P(k) at z = 0 within %.2e" % (cyc, err), flush=True)
     assert err < 1e-6
+
+
+def test_ensemble_of_two_grid_sizes_equals_boxes_alone(gev):
+    """The ensemble driver (one batch per grid size, stepped round-robin on
+    their own streams) gives every box the result of the box run alone."""
+    import ensemble as en
+    tpl = template(gev)
+    specs = en.draw(6, grids=(16, 32), seed=11)
+    with en.Ensemble(specs, tpl, form=gev.FORM_FP32) as E:
+        E.generate()
+        E.run()
+        res = [E.result(s.index) for s in specs]
+    for s, r in zip(specs, res):
+        sim, st, ic, tabs = en.setup(s)
+        with gev.Box(s.ngrid, s.ngrid ** 3) as b64, gev.Box(s.ngrid, s.ngrid ** 3, form=gev.FORM_FP32) as b:
+            b64.generateIC_basic(sim, ic, tpl, *tabs)
+            b.copy_state_from(b64)
+            out = b.run_to_outputs(sim, st, [0.0], s.ngrid // 2, s.boxsize)
+        assert r["cycles"] == st.cycle and r["a"] == st.a and r["z"] == 0.0
+        assert np.array_equal(r["pk"], out["pk"][0]) and np.array_equal(r["occupation"], out["occupation"][0])
