@@ -335,44 +335,53 @@ def serving_report(args):
     plain FP64 runs on the same Poisson arrival trace: request grids from
     {64, 96, 128, 192, 256}^3 (weights .3/.25/.2/.15/.1), half GR and half
     Newtonian, Omega_m and A_s drawn per request, each generated on the
-    device and run from z = 100 to z = 0.  The plain runs take one request at
-    a time on a box of their own; the served runs overlap up to
-    `serving_slots` requests and reuse idle boxes.  Every request's delta
-    power spectrum and phi sum must be identical in both."""
+    device and run from z = 100 to z = 0 with its delta spectrum at z = 0.
+    The plain runs take one request at a time on a box of their own; the
+    served runs are continuously batched (BatchServer: requests of one
+    shape share a batch, join it at the next cycle and leave it when they
+    finish).  Every request's spectrum and phi sum must be identical in
+    both."""
     import gevpm as gev
     import serving as sv
     tpl = gev.load_homogeneous_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
     be = sv.GevBackend(template=tpl)
-    trace = sv.poisson_trace(args.serving_requests, args.serving_rate, 2024)
-    # first use of each grid's kernels and allocations, outside both timings
+    bbe = sv.BatchBackend(template=tpl)
+    rates = [float(v) for v in args.serving_rates.split(",")]
+    traces = {rate: sv.poisson_trace(args.serving_requests, rate, 2024) for rate in rates}
+    # first use of each shape's kernels, allocations and graphs, outside both timings
+    shapes = sorted({(r.ngrid, r.gr) for t in traces.values() for r in t})
+    warm = [sv.Request(-1 - i, g, gr=gr) for i, (g, gr) in enumerate(shapes)]
     with sv.Server(be, sv.default_budget(), slots=1, reuse=False) as s:
-        s.serve([sv.Request(-1 - i, g) for i, g in enumerate(sorted({r.ngrid for r in trace}))])
-    with sv.Server(be, sv.default_budget(), slots=1, reuse=False) as s:
-        plain = s.serve(trace)
-    print("serving: back-to-back plain runs done", file=sys.stderr, flush=True)
-    with sv.Server(be, sv.default_budget(), slots=args.serving_slots, reuse=True) as s:
-        served = s.serve(trace)
-        pool = {"boxes_created": s.created, "boxes_evicted": s.evicted, "peak_running": s.peak_running,
-                "peak_device_bytes": s.peak_in_use}
-    same = all(a.status == b.status == "ok" and a.result["cycles"] == b.result["cycles"]
-               and np.array_equal(a.result["pk"], b.result["pk"]) and a.result["phi_sum"] == b.result["phi_sum"]
-               for a, b in zip(plain, served))
-    ps, pp = sv.summarize(served), sv.summarize(plain)
-    svc = {}
-    for o in plain:
-        r = trace[o.rid]
-        svc.setdefault("%d_%s" % (r.ngrid, "gr" if r.gr else "newton"), []).append(o.service)
-    print("serving: p99 %.3f s served, %.3f s plain" % (ps["p99_s"], pp["p99_s"]), file=sys.stderr, flush=True)
-    return {"trace": {"requests": len(trace), "rate_per_s": args.serving_rate, "seed": 2024,
+        s.serve(warm)
+    with sv.BatchServer(bbe, sv.default_budget()) as s:
+        s.serve(warm)
+    runs = []
+    for rate, trace in traces.items():
+        with sv.Server(be, sv.default_budget(), slots=1, reuse=False) as s:
+            plain = s.serve(trace)
+        with sv.BatchServer(bbe, sv.default_budget()) as s:
+            served = s.serve(trace)
+            pool = {"batches_created": s.created, "batches_evicted": s.evicted, "peak_running": s.peak_running,
+                    "peak_device_bytes": s.peak_in_use, "rounds": s.rounds}
+        same = all(a.status == b.status == "ok" and a.result["cycles"] == b.result["cycles"]
+                   and np.array_equal(a.result["pk"], b.result["pk"]) and a.result["phi_sum"] == b.result["phi_sum"]
+                   for a, b in zip(plain, served))
+        ps, pp = sv.summarize(served), sv.summarize(plain)
+        print("serving at %.0f/s: p99 %.3f s served, %.3f s plain; %.3e vs %.3e particle-steps/s"
+              % (rate, ps["p99_s"], pp["p99_s"], ps["particle_steps_per_s"] or 0, pp["particle_steps_per_s"] or 0),
+              file=sys.stderr, flush=True)
+        runs.append({"rate_per_s": rate, "served": ps, "back_to_back_plain": pp,
+                     "p99_speedup_vs_plain": round(pp["p99_s"] / ps["p99_s"], 3) if ps["p99_s"] else None,
+                     "throughput_ratio_vs_plain": (round(ps["particle_steps_per_s"] / pp["particle_steps_per_s"], 3)
+                                                   if ps["particle_steps_per_s"] and pp["particle_steps_per_s"]
+                                                   else None),
+                     "identical_results_alone_and_in_the_mix": bool(same), "pool": pool})
+    return {"trace": {"requests": args.serving_requests, "seed": 2024,
                       "grids": [64, 96, 128, 192, 256], "weights": [0.3, 0.25, 0.2, 0.15, 0.1],
                       "gr_fraction": 0.5, "z": "100 -> 0", "form": "fp64"},
-            "slots": args.serving_slots, "served": ps, "back_to_back_plain": pp,
-            "p99_speedup_vs_plain": round(pp["p99_s"] / ps["p99_s"], 3) if ps["p99_s"] else None,
-            "throughput_ratio_vs_plain": (round(ps["particle_steps_per_s"] / pp["particle_steps_per_s"], 3)
-                                          if ps["particle_steps_per_s"] and pp["particle_steps_per_s"] else None),
-            "identical_results_alone_and_in_the_mix": bool(same),
-            "plain_service_s_mean": {k: round(float(np.mean(v)), 4) for k, v in sorted(svc.items())},
-            "pool": pool}
+            "server": "continuous batching (one batch per grid and gravity, slots 8/4/2 by grid)",
+            "plain": "back-to-back: one request at a time on a box of its own (gev_run_to_outputs)",
+            "runs": runs}
 
 
 def ensemble_report(args):
@@ -456,11 +465,10 @@ def main():
                     help="skip the FP32 form's timing and error report")
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the whole run from z = 100 to z = 0 (both forms)")
-    ap.add_argument("--serving", action="store_true",
-                    help="also serve a Poisson trace of whole simulations against back-to-back plain runs")
-    ap.add_argument("--serving-requests", type=int, default=32)
-    ap.add_argument("--serving-rate", type=float, default=6.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip serving Poisson traces of whole simulations against back-to-back plain runs")
+    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-rates", default="12,48", help="Poisson arrival rates, requests/s")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble report")
     ap.add_argument("--ensemble-boxes", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -583,7 +591,7 @@ def main():
     box.close()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_report(args)
-    if args.serving:
+    if not args.no_serving:
         out["serving"] = serving_report(args)
     print(json.dumps(out))
 

@@ -415,21 +415,28 @@ int verify_scalars(gev_batch *B) {
     return 0;
 }
 
-// maxvel of every running box from its kick maximum (main.cpp:816-822)
-int read_maxvel(gev_batch *B) {
+// maxvel of box b from its last kick's maximum (main.cpp:816-822)
+int read_maxvel_box(gev_batch *B, int b) {
     gev_ctx *c = B->base;
-    for (int b = 0; b < c->nbox; b++) {
-        BoxHost &h = B->box[b];
-        if (!h.running || h.st.cycle == 0) continue;
-        unsigned long long vb;
-        int rc = batch_hip(B, hipMemcpy(&vb, box_ptr(c->red_u, c->stride, b), sizeof vb, hipMemcpyDeviceToHost),
-                           "vmax readback");
+    BoxHost &h = B->box[b];
+    if (!h.running || h.st.cycle == 0) return 0;
+    unsigned long long vb;
+    int rc = batch_hip(B, hipMemcpyAsync(&vb, box_ptr(c->red_u, c->stride, b), sizeof vb, hipMemcpyDeviceToHost,
+                                         c->stream), "vmax readback");
+    if (!rc) rc = batch_hip(B, hipStreamSynchronize(c->stream), "vmax sync");
+    if (rc) return rc;
+    double mv;
+    memcpy(&mv, &vb, sizeof mv);
+    mv = sqrt(mv);
+    if (h.p.gr_flag > 0) mv /= sqrt(mv * mv + 1.0);
+    h.st.maxvel = mv;
+    return 0;
+}
+
+int read_maxvel(gev_batch *B) {
+    for (int b = 0; b < (int)B->box.size(); b++) {
+        int rc = read_maxvel_box(B, b);
         if (rc) return rc;
-        double mv;
-        memcpy(&mv, &vb, sizeof mv);
-        mv = sqrt(mv);
-        if (h.p.gr_flag > 0) mv /= sqrt(mv * mv + 1.0);
-        h.st.maxvel = mv;
     }
     return 0;
 }
@@ -595,6 +602,9 @@ GEV_EXPORT int gev_batch_step_outputs(gev_batch *B, int32_t use_graph, int32_t *
     std::vector<bool> before(nbox);
     for (int b = 0; b < nbox; b++) before[b] = B->box[b].finished;
     int rc = any_running(B) ? batch_cycle(B, use_graph != 0, true) : 0;
+    if (rc) return rc;
+    for (int b = 0; b < nbox && !rc; b++)
+        if (B->box[b].finished && !before[b]) rc = read_maxvel_box(B, b);
     if (rc) return rc;
     if (finished)
         for (int b = 0; b < nbox; b++) finished[b] = B->box[b].finished && !before[b];

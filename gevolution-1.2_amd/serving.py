@@ -22,13 +22,23 @@ Residency and slot reuse.  A finished box goes back to a pool keyed by
 conditions overwrite the whole state, so nothing of the previous run
 survives, and the allocation is saved.
 
-Overlap.  Each admitted request runs in a host thread on its own box, and
-so on its own HIP stream; ctypes releases the interpreter lock inside
-library calls, so the streams of different requests overlap on the device.
+Two servers share that admission policy:
+
+- Server runs each admitted request in a host thread on a box of its own
+  (its own HIP stream).  With one slot it is the back-to-back plain run.
+- BatchServer is continuous batching: requests of the same grid and gravity
+  share a batch (include/gev_pm.h gev_batch_*), whose every cycle kernel is
+  one launch over all the boxes it holds.  A request that arrives while
+  others run takes a free slot of a batch of its shape (or a new batch, if
+  the memory budget allows) and joins the next cycle; a request that writes
+  its z = 0 spectrum leaves its slot at once.  All running requests advance
+  one cycle per round, the batches on their own streams.
+
 A request's results do not depend on what else runs: its kernels touch only
-its own box, in a fixed order, and every atomic in them either only counts
-(the filing, whose order a sort then fixes) or is order-independent
-(integer maxima, fixed-point sums).
+its own box (its slab of a batch), in a fixed order, and every atomic in
+them either only counts (the filing, whose order a sort then fixes) or is
+order-independent (integer maxima, fixed-point sums); a batch box equals the
+box run alone bit for bit.
 """
 import collections
 import math
@@ -321,6 +331,183 @@ class GevBackend:
         return {"a": st.a, "cycles": st.cycle, "particle_steps": req.npart * st.cycle, "z": float(out["z"][0]),
                 "k": out["k"][0], "pk": out["pk"][0], "occupation": out["occupation"][0], "phi_sum": phi_sum,
                 "maxvel": st.maxvel}
+
+
+class BatchBackend:
+    """Batches on the current GPU through the C ABI, for BatchServer."""
+
+    def __init__(self, template=None, numbins=None, form=gevpm.FORM_FP64, graph=True):
+        self.gev = GevBackend(template, numbins)
+        self.form = form
+        self.graph = graph
+
+    def key(self, req):
+        return (req.ngrid, 1 if req.gr else 0, self.form)
+
+    def slot_footprint(self, req):
+        if req.ngrid % self.gev.tile:
+            return -1
+        return gevpm.batch_footprint_bytes(req.ngrid, req.npart, 1, self.form, 1 if req.gr else 0)
+
+    def create(self, key, slots):
+        N, gr, form = key
+        B = gevpm.Batch(N, N ** 3, slots, gr=gr, form=form)
+        B.set_outputs([0.0], self.gev.numbins or N // 2)
+        return B
+
+    def destroy(self, batch):
+        batch.close()
+
+    def start(self, batch, slot, req):
+        """The request's initial conditions into the slot; the slot joins the
+        batch's next cycle."""
+        N = req.ngrid
+        sim, st = GevBackend.setup(req)
+        L, h = req.box_mpc, req.h
+        k, d, t = self.gev._bbks.bbks_table(self.gev.kgrid, req.z_in, req.Omega_m, h)
+        ic = gevpm.ICParams(int(req.seed), 1, 1, N // self.gev.tile, req.z_in, req.A_s, req.n_s, 0.05, L)
+        view = batch.box(slot)
+        if self.form == gevpm.FORM_FP64:
+            view.generateIC_basic(sim, ic, self.gev.template, k * L, d, t * L / h, d, t * L / h)
+        else:
+            with gevpm.Box(N, N ** 3) as tmp:
+                tmp.generateIC_basic(sim, ic, self.gev.template, k * L, d, t * L / h, d, t * L / h)
+                view.copy_state_from(tmp)
+        batch.start_box(slot, sim, st, L)
+
+    def step(self, batch):
+        return batch.step(self.graph)
+
+    def result(self, batch, slot, req):
+        out = batch.outputs(slot)
+        st = batch.state(slot)
+        return {"a": st.a, "cycles": st.cycle, "particle_steps": req.npart * st.cycle, "z": float(out["z"][0]),
+                "k": out["k"][0], "pk": out["pk"][0], "occupation": out["occupation"][0],
+                "phi_sum": batch.box(slot).field_sum(gevpm.PHI), "maxvel": st.maxvel}
+
+    def release(self, batch, slot):
+        batch.stop_box(slot)
+
+
+class BatchServer:
+    """Continuous batching over a backend with key(req), slot_footprint(req),
+    create(key, slots), destroy(batch), start(batch, slot, req),
+    step(batch) -> finished slots, result(batch, slot, req) and
+    release(batch, slot).  slots(key) gives a new batch's slot count."""
+
+    def __init__(self, backend, memory_budget, slots=None, clock=time.perf_counter):
+        self.backend = backend
+        self.budget = int(memory_budget)
+        self.slots = slots or (lambda key: 8 if key[0] <= 128 else (4 if key[0] <= 192 else 2))
+        self.clock = clock
+        self.batches = []          # [key, batch, slots, {slot: (req, admitted)}, bytes]
+        self.in_use = 0
+        self.peak_in_use = 0
+        self.peak_running = 0
+        self.created = 0
+        self.evicted = 0
+        self.rounds = 0
+
+    def close(self):
+        for e in self.batches:
+            self.backend.destroy(e[1])
+            self.in_use -= e[4]
+        self.batches = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _slot_for(self, req):
+        """(entry, slot) for req, a new batch if needed and affordable, or None."""
+        key = self.backend.key(req)
+        for e in self.batches:
+            if e[0] == key and len(e[3]) < e[2]:
+                return e, min(set(range(e[2])) - set(e[3]))
+        n = self.slots(key)
+        need = self.backend.slot_footprint(req) * n
+        while self.in_use + need > self.budget:
+            idle = [e for e in self.batches if not e[3]]
+            if not idle:
+                return None
+            e = idle[0]
+            self.backend.destroy(e[1])
+            self.batches.remove(e)
+            self.in_use -= e[4]
+            self.evicted += 1
+        e = [key, self.backend.create(key, n), n, {}, need]
+        self.batches.append(e)
+        self.created += 1
+        self.in_use += need
+        self.peak_in_use = max(self.peak_in_use, self.in_use)
+        return e, 0
+
+    def serve(self, requests):
+        """Runs the requests as they arrive; outcomes in request order."""
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        outcomes, waiting = {}, collections.deque()
+        t0 = self.clock()
+        i = 0
+        while i < len(reqs) or waiting or any(e[3] for e in self.batches):
+            now = self.clock() - t0
+            while i < len(reqs) and reqs[i].arrival <= now:
+                waiting.append(reqs[i])
+                i += 1
+            while waiting:
+                r = waiting[0]
+                one = self.backend.slot_footprint(r)
+                if one < 0 or one > self.budget:
+                    waiting.popleft()
+                    outcomes[r.rid] = Outcome(r.rid, "rejected", r.arrival,
+                                              error="footprint %d B exceeds the budget %d B" % (one, self.budget))
+                    continue
+                got = self._slot_for(r)
+                if got is None:
+                    if not any(e[3] for e in self.batches):
+                        waiting.popleft()
+                        outcomes[r.rid] = Outcome(r.rid, "rejected", r.arrival,
+                                                  error="a batch for it does not fit next to %d B in use" % self.in_use)
+                        continue
+                    break  # waits for a slot or memory; arrival order is kept
+                waiting.popleft()
+                e, slot = got
+                try:
+                    self.backend.start(e[1], slot, r)
+                except Exception as ex:
+                    outcomes[r.rid] = Outcome(r.rid, "failed", r.arrival, error="start: %s" % ex)
+                    continue
+                e[3][slot] = (r, self.clock() - t0)
+            running = sum(len(e[3]) for e in self.batches)
+            self.peak_running = max(self.peak_running, running)
+            if not running:
+                if i < len(reqs):
+                    time.sleep(max(0.0, reqs[i].arrival - (self.clock() - t0)))
+                continue
+            self.rounds += 1
+            broken = []
+            for e in self.batches:
+                if not e[3]:
+                    continue
+                try:
+                    fin = self.backend.step(e[1])
+                except Exception as ex:  # the batch is lost: fail its requests, free it
+                    for slot, (r, adm) in e[3].items():
+                        outcomes[r.rid] = Outcome(r.rid, "failed", r.arrival, adm, self.clock() - t0, error=str(ex))
+                    e[3].clear()
+                    broken.append(e)
+                    continue
+                for slot in fin:
+                    r, adm = e[3].pop(slot)
+                    res = self.backend.result(e[1], slot, r)
+                    self.backend.release(e[1], slot)
+                    outcomes[r.rid] = Outcome(r.rid, "ok", r.arrival, adm, self.clock() - t0, result=res)
+            for e in broken:
+                self.backend.destroy(e[1])
+                self.batches.remove(e)
+                self.in_use -= e[4]
+        return [outcomes[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
 
 
 def default_budget():

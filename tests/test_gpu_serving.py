@@ -93,3 +93,24 @@ def test_served_run_matches_the_oracle():
     err = np.max(np.abs(res["pk"] - want["pk"][0])[live] / want["pk"][0][live])
     print("served 32^3 GR run: %d cycles, P(k) at z = 0 within %.2e of the oracle" % (st.cycle, err), flush=True)
     assert err < 1e-6
+
+
+def test_continuous_batching_gives_every_request_its_plain_result():
+    """BatchServer (requests of one shape share a batch and join it while
+    others run) against each request run alone on a box of its own: the
+    same cycles, phi sum and z = 0 spectrum, bit for bit."""
+    import gevpm as gev
+    import serving as sv
+    tpl = gev.load_homogeneous_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
+    reqs = _requests()
+    for k, r in enumerate(reqs):
+        r.arrival = 0.01 * k  # later requests arrive while earlier ones run
+    with sv.BatchServer(sv.BatchBackend(template=tpl), sv.default_budget(), slots=lambda key: 2) as s:
+        served = s.serve(reqs)
+        assert s.peak_running >= 3 and s.rounds > 100
+    be = sv.GevBackend(template=tpl)
+    for o, r in zip(served, reqs):
+        assert o.status == "ok", o.error
+        with gev.Box(r.ngrid, r.npart) as b:
+            alone = be.run(b, r)
+        _same(o.result, alone)

@@ -248,3 +248,98 @@ def test_a_request_that_cannot_fit_beside_held_memory_is_rejected_not_waited_for
         out = s.serve(reqs([16]))
         s.in_use = 0
     assert out[0].status == "rejected" and "does not fit" in out[0].error
+
+
+class FakeBatchBackend:
+    """Batches whose slots finish a request after N // 8 steps."""
+
+    def __init__(self, fail_key=None, step_s=0.005):
+        self.live = set()
+        self.fail_key = fail_key
+        self.step_s = step_s
+        self.next = 0
+        self.starts = []
+
+    def key(self, r):
+        return (r.ngrid, 1 if r.gr else 0, 0)
+
+    def slot_footprint(self, r):
+        return r.ngrid ** 3
+
+    def create(self, key, slots):
+        self.next += 1
+        b = {"id": self.next, "key": key, "left": {}, "slots": slots}
+        self.live.add(self.next)
+        return b
+
+    def destroy(self, b):
+        self.live.discard(b["id"])
+
+    def start(self, b, slot, r):
+        assert slot not in b["left"] and 0 <= slot < b["slots"]
+        b["left"][slot] = r.ngrid // 8
+        self.starts.append((r.rid, b["id"], slot))
+
+    def step(self, b):
+        if b["key"] == self.fail_key:
+            raise RuntimeError("injected batch failure")
+        time.sleep(self.step_s)
+        fin = []
+        for slot in list(b["left"]):
+            b["left"][slot] -= 1
+            if b["left"][slot] == 0:
+                fin.append(slot)
+        return fin
+
+    def result(self, b, slot, r):
+        return {"particle_steps": r.ngrid ** 3 * (r.ngrid // 8), "batch": b["id"], "slot": slot}
+
+    def release(self, b, slot):
+        del b["left"][slot]
+
+
+def test_batch_server_shares_batches_and_admits_while_others_run():
+    be = FakeBatchBackend()
+    arr = [0.0, 0.0, 0.0, 0.02]
+    rs = [sv.Request(i, 64, arrival=t) for i, t in enumerate(arr)]
+    with sv.BatchServer(be, 10 ** 9, slots=lambda k: 4) as s:
+        out = s.serve(rs)
+        assert all(o.status == "ok" for o in out)
+        assert s.created == 1 and len({o.result["batch"] for o in out}) == 1
+        # the late request joined while the first three were still running
+        assert out[3].admitted < min(o.finished for o in out[:3])
+        assert s.peak_running == 4
+    assert not be.live
+
+
+def test_batch_server_frees_slots_for_waiting_requests_and_keeps_order():
+    be = FakeBatchBackend()
+    rs = [sv.Request(i, g) for i, g in enumerate([32, 64, 32, 32, 64])]
+    budget = 2 * 32 ** 3 + 2 * 64 ** 3  # one two-slot batch of each shape
+    with sv.BatchServer(be, budget, slots=lambda k: 2) as s:
+        out = s.serve(rs)
+        assert all(o.status == "ok" for o in out)
+        assert s.peak_running <= 4
+    adm = [o.admitted for o in out]
+    assert adm == sorted(adm)
+    # the third 32^3 request took the slot the first one left
+    slots = {rid: (b, sl) for rid, b, sl in be.starts}
+    assert slots[3] == slots[0] or slots[3] == slots[2]
+
+
+def test_batch_server_budget_eviction_and_rejection():
+    be = FakeBatchBackend()
+    budget = 2 * 32 ** 3  # one two-slot batch of 32^3, or one of 16^3
+    with sv.BatchServer(be, budget, slots=lambda k: 2) as s:
+        out = s.serve([sv.Request(0, 32), sv.Request(1, 16), sv.Request(2, 128), sv.Request(3, 32)])
+        assert [o.status for o in out] == ["ok", "ok", "rejected", "ok"]
+        assert s.evicted >= 1 and s.peak_in_use <= budget
+    assert not be.live
+
+
+def test_batch_server_failed_batch_fails_its_requests_only():
+    be = FakeBatchBackend(fail_key=(16, 1, 0))
+    with sv.BatchServer(be, 10 ** 9, slots=lambda k: 2) as s:
+        out = s.serve([sv.Request(0, 16), sv.Request(1, 32), sv.Request(2, 16)])
+    assert [o.status for o in out] == ["failed", "ok", "failed"]
+    assert "injected" in out[0].error and not be.live
