@@ -406,6 +406,7 @@ def ensemble_report(args):
         t_ic = time.perf_counter() - t0
         print("ensemble: %d boxes generated in %.2f s (%.1f GB)" % (len(specs), t_ic, E.footprint_bytes() / 1e9),
               file=sys.stderr, flush=True)
+        tuned = E.autotune()
         t_run = E.run(graph=True)
         res = [E.result(s.index) for s in specs]
     steps = sum(r["particle_steps"] for r in res)
@@ -441,6 +442,7 @@ def ensemble_report(args):
                          "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(nbytes)},
             "boxes_alone_s": round(t_alone, 3), "speedup_vs_alone": round(t_alone / t_run, 3),
             "identical_to_alone_sample": bool(same), "per_grid": per_n,
+            "autotuned_fft_tiles": {str(N): t for N, t in tuned.items()},
             "cycles": sorted({r["cycles"] for r in res})}
 
 

@@ -26,7 +26,10 @@
 // stops there and the others go on.
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "gev_internal.h"
@@ -180,6 +183,10 @@ struct gev_batch {
     int numbins = 0;
     // the two stages captured once each
     hipGraphExec_t g_fields = nullptr, g_particles = nullptr;
+    // the autotuner's pick: row bytes of the y and z FFT pass tiles and the
+    // pass times (ms per launch over every box) of each candidate
+    bool tuned = false;
+    float tune_ms[2][3] = {{0.f}};
     std::string err;
     int fail(int code, const std::string &m) {
         err = m;
@@ -329,6 +336,90 @@ int run_stage(gev_batch *B, bool use_graph, bool fields) {
     return batch_hip(B, hipGraphLaunch(g, B->base->stream), "graph launch");
 }
 
+// ---------------------------------------------------------------- autotuner
+//
+// The strided FFT passes (four y and four z passes per cycle and component
+// class) transform N-point columns staged W at a time; W (the tile's row
+// bytes) trades the occupancy of a workgroup against whole cache lines per
+// row, and the best choice depends on N and on how many boxes share the
+// launch.  The tuner times each candidate tile (64-, 128- and 256-byte rows)
+// on the batch's own FFT scratch field over every box, and keeps the
+// fastest per axis.  Every column is transformed the same way whatever the
+// tile, so the pick changes no result.  Picks are kept per (N, form, nbox)
+// for the process.
+
+constexpr int TUNE_RB[3] = {64, 128, 256};
+
+std::mutex tune_mutex;
+std::map<std::tuple<int, int, int>, std::pair<int, int>> tune_cache;
+
+int autotune(gev_batch *B) {
+    gev_ctx *c = B->base;
+    B->tuned = true;
+    if (c->N > 256) return 0;  // the tile variants exist for N <= 256
+    const auto key = std::make_tuple(c->N, (int)c->fp32, c->nbox);
+    {
+        std::lock_guard<std::mutex> lock(tune_mutex);
+        auto it = tune_cache.find(key);
+        if (it != tune_cache.end()) {
+            c->col_rb[0] = it->second.first;
+            c->col_rb[1] = it->second.second;
+            return 0;
+        }
+    }
+    // every box on for the timing launches
+    BoxCycle *on = nullptr;
+    int rc = batch_hip(B, hipMalloc(&on, sizeof(BoxCycle) * c->nbox), "tuner flags");
+    std::vector<BoxCycle> h(c->nbox);
+    for (auto &y : h) {
+        memset(&y, 0, sizeof y);
+        y.on = y.on_phi = 1;
+    }
+    if (!rc) rc = batch_hip(B, hipMemcpy(on, h.data(), sizeof(BoxCycle) * c->nbox, hipMemcpyHostToDevice),
+                            "tuner flags");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = batch_hip(B, hipEventCreate(&e0), "tuner event");
+    if (!rc) rc = batch_hip(B, hipEventCreate(&e1), "tuner event");
+    const Batch all{c->stride, on, 0};
+    const bool timing = c->timing;
+    c->timing = false;
+    int best[2] = {0, 0};
+    for (int axis = 0; axis < 2 && !rc; axis++) {
+        float best_ms = 0.f;
+        for (int v = 0; v < 3 && !rc; v++) {
+            c->col_rb[axis] = TUNE_RB[v];
+            auto pass = [&]() {
+                return axis == 0 ? fft_forward_y(c, c->fft_tmp, 1, all) : fft_forward_z(c, c->fft_tmp, 1, all);
+            };
+            rc = pass();  // first use of the variant, untimed
+            if (!rc) rc = batch_hip(B, hipEventRecord(e0, c->stream), "tuner event");
+            for (int r = 0; r < 3 && !rc; r++) rc = pass();
+            if (!rc) rc = batch_hip(B, hipEventRecord(e1, c->stream), "tuner event");
+            if (!rc) rc = batch_hip(B, hipEventSynchronize(e1), "tuner sync");
+            float ms = 0.f;
+            if (!rc) rc = batch_hip(B, hipEventElapsedTime(&ms, e0, e1), "tuner time");
+            ms /= 3.f;
+            B->tune_ms[axis][v] = ms;
+            if (!rc && (v == 0 || ms < best_ms)) {
+                best_ms = ms;
+                best[axis] = TUNE_RB[v];
+            }
+        }
+        c->col_rb[axis] = best[axis];
+    }
+    c->timing = timing;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (on) (void)hipFree(on);
+    if (rc) {
+        c->col_rb[0] = c->col_rb[1] = 0;
+        return rc;
+    }
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    tune_cache[key] = {best[0], best[1]};
+    return 0;
+}
+
 bool any_running(const gev_batch *B) {
     for (auto &h : B->box)
         if (h.running && !h.finished) return true;
@@ -340,6 +431,7 @@ bool any_running(const gev_batch *B) {
 int batch_cycle(gev_batch *B, bool use_graph, bool outputs) {
     gev_ctx *c = B->base;
     int rc = 0;
+    if (!B->tuned && (rc = autotune(B))) return rc;
     sync_views(B);
     if (B->dirty && (rc = upload_scalars(B))) return rc;
     if ((rc = run_stage(B, use_graph, true))) return rc;
@@ -626,6 +718,16 @@ GEV_EXPORT int gev_batch_synchronize(gev_batch *B) {
     if (!B) return GEV_EINVAL;
     int rc = verify_scalars(B);
     return rc ? rc : read_maxvel(B);
+}
+
+GEV_EXPORT int gev_batch_autotune(gev_batch *B, int32_t *row_bytes_y, int32_t *row_bytes_z, float *pass_ms) {
+    if (!B) return GEV_EINVAL;
+    int rc = B->tuned ? 0 : autotune(B);
+    if (rc) return rc;
+    if (row_bytes_y) *row_bytes_y = B->base->col_rb[0];
+    if (row_bytes_z) *row_bytes_z = B->base->col_rb[1];
+    if (pass_ms) memcpy(pass_ms, B->tune_ms, sizeof B->tune_ms);
+    return 0;
 }
 
 GEV_EXPORT int gev_batch_get_outputs(gev_batch *B, int32_t b, double *k, double *pk, double *sigma_k, double *sigma_pk,

@@ -143,8 +143,10 @@ __global__ __launch_bounds__(TPB) void k_c2r_x(const typename cplx_of<T>::type *
 }
 
 // c2c along a strided axis.  Column c = (blockIdx.y, kx0 + w): elements at
-// in[blockIdx.y*os + kx + e*es], e < N.  INV = backward (sign +1).
-template <typename T, int N, bool INV>
+// in[blockIdx.y*os + kx + e*es], e < N.  INV = backward (sign +1).  RB: the
+// tile's row bytes (0: the form's default; every column is transformed the
+// same way whatever the tile).
+template <typename T, int N, bool INV, int RB = 0>
 __global__ __launch_bounds__(col_tpb<T>()) void k_c2c_cols(const typename cplx_of<T>::type *in,
                                                   typename cplx_of<T>::type *out, int H,
                                                   int64_t es, int64_t os,
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(col_tpb<T>()) void k_c2c_cols(const typename cplx_o
     if (!gev::box_on(bt, blockIdx.z)) return;
     in += blockIdx.z * cs;  // component (or box of a batch) blockIdx.z
     out += blockIdx.z * cs;
-    constexpr int W = col_tile<T, N>();
+    constexpr int W = col_tile<T, N, RB>();
     __shared__ C buf[N * W];
     __shared__ C tw[N];
     const int kx0 = blockIdx.x * W;
@@ -204,6 +206,31 @@ const char *pass_name(int pass, int n) {
     return names[pass][n - 1];
 }
 
+// One strided pass (axis: 0 = y, 1 = z) with the context's tile for that
+// axis: ctx->col_rb[axis] row bytes (0: the default).  The row variants
+// exist for N <= 256, the grids batches run.
+template <typename T, int N, bool INV>
+void launch_cols(gev_ctx *ctx, int axis, unsigned nb, const typename cplx_of<T>::type *in,
+                 typename cplx_of<T>::type *out, int H, int64_t es, int64_t os,
+                 const typename cplx_of<T>::type *tw, int64_t cs, const gev::Batch &bt) {
+    const int rb = N <= 256 ? ctx->col_rb[axis] : 0;
+#define GEV_COLS(R)                                                                                             \
+    hipLaunchKernelGGL((k_c2c_cols<T, N, INV, R>), dim3((H + col_tile<T, N, R>() - 1) / col_tile<T, N, R>(), N, nb), \
+                       dim3(col_tpb<T>()), 0, ctx->stream, in, out, H, es, os, tw, cs, bt)
+    if constexpr (N <= 256) {
+        if (rb == 64) {
+            GEV_COLS(64);
+            return;
+        }
+        if (rb == 256) {
+            GEV_COLS(256);
+            return;
+        }
+    }
+    GEV_COLS(0);
+#undef GEV_COLS
+}
+
 // passes: bit 0 = x (r2c), bit 1 = y, bit 2 = z, all components pass by pass.
 // The x pass is one launch over up to MAXC components (blockIdx.z).  Over the
 // boxes of a batch (bt.cyc set) every pass runs one launch per component with
@@ -216,7 +243,6 @@ int run_forward(gev_ctx *ctx, const T *real, typename cplx_of<T>::type *k, int n
     const int64_t nrows = (int64_t)N * N;
     const int64_t n3v = (int64_t)N * N * N, nkv = (int64_t)P * N * N;
     constexpr int LINES = x_lines<N>();
-    constexpr int W = col_tile<T, N>();
     const bool batch = bt.cyc != nullptr;
     const unsigned nb = batch ? (unsigned)ctx->nbox : 1u;
     const int64_t rs = batch ? ctx->stride / (int64_t)sizeof(T) : 0, ks = batch ? ctx->stride / (int64_t)sizeof(C) : 0;
@@ -237,13 +263,11 @@ int run_forward(gev_ctx *ctx, const T *real, typename cplx_of<T>::type *k, int n
         // than one launch over the components)
         for (int c = c0; c < c0 + nc && (passes & 2); c++) {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Y, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
-                               ctx->stream, k + c * nkv, k + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
+            launch_cols<T, N, false>(ctx, 0, nb, k + c * nkv, k + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
         }
         for (int c = c0; c < c0 + nc && (passes & 4); c++) {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Z, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, false>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
-                               ctx->stream, k + c * nkv, k + c * nkv, H, (int64_t)P * N, (int64_t)P, tw, ks, bt);
+            launch_cols<T, N, false>(ctx, 1, nb, k + c * nkv, k + c * nkv, H, (int64_t)P * N, (int64_t)P, tw, ks, bt);
         }
     }
     return gev::check_hip(ctx, hipGetLastError(), "fft forward launch");
@@ -261,7 +285,6 @@ int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int 
     const int64_t nrows = (int64_t)N * N;
     const int64_t n3v = (int64_t)N * N * N, nkv = (int64_t)P * N * N;
     constexpr int LINES = x_lines<N>();
-    constexpr int W = col_tile<T, N>();
     const bool batch = bt.cyc != nullptr;
     const unsigned nb = batch ? (unsigned)ctx->nbox : 1u;
     const int64_t rs = batch ? ctx->stride / (int64_t)sizeof(T) : 0, ks = batch ? ctx->stride / (int64_t)sizeof(C) : 0;
@@ -271,8 +294,8 @@ int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int 
             const int nc = ncomp - c0 < MAXC ? ncomp - c0 : MAXC;
             for (int c = c0; c < c0 + nc; c++) {
                 gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
-                hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
-                                   ctx->stream, kk + c * nkv, kk + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
+                launch_cols<T, N, true>(ctx, 0, nb, kk + c * nkv, kk + c * nkv, H, (int64_t)P, (int64_t)P * N, tw, ks,
+                                        bt);
             }
             {
                 gev::ScopedTimer t(ctx, pass_name(P_C2R_X, batch ? 1 : nc));
@@ -291,13 +314,11 @@ int run_backward(gev_ctx *ctx, const typename cplx_of<T>::type *k, T *real, int 
     for (int c = 0; c < ncomp; c++) {
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Z_INV, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
-                               ctx->stream, k + c * nkv, tmp, H, (int64_t)P * N, (int64_t)P, tw, ks, bt);
+            launch_cols<T, N, true>(ctx, 1, nb, k + c * nkv, tmp, H, (int64_t)P * N, (int64_t)P, tw, ks, bt);
         }
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2C_Y_INV, 1));
-            hipLaunchKernelGGL((k_c2c_cols<T, N, true>), dim3((H + W - 1) / W, N, nb), dim3(col_tpb<T>()), 0,
-                               ctx->stream, tmp, tmp, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
+            launch_cols<T, N, true>(ctx, 0, nb, tmp, tmp, H, (int64_t)P, (int64_t)P * N, tw, ks, bt);
         }
         {
             gev::ScopedTimer t(ctx, pass_name(P_C2R_X, 1));
@@ -369,6 +390,10 @@ int fft_forward_xy(gev_ctx *ctx, const double *real, double2 *kfield, int ncomp,
 
 int fft_forward_y(gev_ctx *ctx, double2 *kfield, int ncomp, const Batch &bt) {
     return forward(ctx, nullptr, kfield, ncomp, 2, bt);
+}
+
+int fft_forward_z(gev_ctx *ctx, double2 *kfield, int ncomp, const Batch &bt) {
+    return forward(ctx, nullptr, kfield, ncomp, 4, bt);
 }
 
 int fft_backward_yx(gev_ctx *ctx, double2 *kfield, double *real, int ncomp, const Batch &bt) {

@@ -178,8 +178,10 @@ template <int N> constexpr int x_lines() {
 #define GEV_COL_TPB_F32 512
 #endif
 template <typename T> constexpr int col_tpb() { return sizeof(T) == 4 ? GEV_COL_TPB_F32 : GEV_COL_TPB; }
-template <typename T, int N> constexpr int col_tile() {
-    int w = (sizeof(T) == 4 ? GEV_COL_BYTES_F32 : GEV_COL_BYTES) / (int)(2 * sizeof(T));
+// RB: row bytes of a tile (0: the default of the form); the autotuner of a
+// batch may pick 64 or 256 instead
+template <typename T, int N, int RB = 0> constexpr int col_tile() {
+    int w = (RB ? RB : (sizeof(T) == 4 ? GEV_COL_BYTES_F32 : GEV_COL_BYTES)) / (int)(2 * sizeof(T));
     while (w > 1 && w * N * (int)(2 * sizeof(T)) > 65536) w /= 2;
     return w;
 }

@@ -161,6 +161,10 @@ struct gev_ctx {
     uint32_t *sort_segs = nullptr, *sort_big = nullptr, *sort_n = nullptr;
     int64_t scan_nblk = 0;
 
+    // row bytes of the y (0) and z (1) FFT pass tiles; 0: the form's default
+    // (a batch's autotuner sets them)
+    int col_rb[2] = {0, 0};
+
     // power-spectrum scratch (tables, workgroup histograms, bin totals) for
     // up to pk_bins bins
     double *pk_scratch = nullptr;
@@ -243,6 +247,7 @@ int launch_x_prep_tensor(gev_ctx *c, double coeff, const Batch &bt = Batch{});
 int launch_x_prep_scalar(gev_ctx *c, double bg, double coeff, double coeff2, double coeff3,
                          const Batch &bt = Batch{});
 int fft_forward_y(gev_ctx *ctx, double2 *kfield, int ncomp, const Batch &bt = Batch{});  // the y pass alone, in place
+int fft_forward_z(gev_ctx *ctx, double2 *kfield, int ncomp, const Batch &bt = Batch{});  // the z pass alone, in place
 // z pass + solve + inverse z pass (fftsolve.hip); kstore / k0 optional
 int launch_zsolve_poisson(gev_ctx *c, const double2 *in, double2 *out, double coeff, double modif,
                           double2 *kstore, double2 *k0, const Batch &bt = Batch{});

@@ -124,6 +124,10 @@ class Ensemble:
                 if tmp is not None:
                     tmp.close()
 
+    def autotune(self):
+        """Each batch's tile autotuner (outside the run): {N: picks}."""
+        return {N: B.autotune() for N, (B, _) in sorted(self.batches.items())}
+
     def run(self, graph=True):
         """Every box to z_end; returns the wall time in seconds (from the
         first launch to the last box's output)."""

@@ -51,7 +51,7 @@ EXPORTED = [
     "gev_batch_footprint_bytes", "gev_batch_create", "gev_batch_destroy", "gev_batch_last_error", "gev_batch_box",
     "gev_batch_start_box", "gev_batch_stop_box", "gev_batch_get_state", "gev_batch_run_cycles",
     "gev_batch_set_outputs", "gev_batch_step_outputs", "gev_batch_run_to_outputs", "gev_batch_get_outputs",
-    "gev_batch_synchronize",
+    "gev_batch_synchronize", "gev_batch_autotune",
 ]
 
 
@@ -195,6 +195,7 @@ def lib():
         L.gev_batch_run_to_outputs.argtypes = [_p, _i32, _i32]
         L.gev_batch_get_outputs.argtypes = [_p, _i32, _p, _p, _p, _p, _p, _p]
         L.gev_batch_synchronize.argtypes = [_p]
+        L.gev_batch_autotune.argtypes = [_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32), _p]
         L.gev_run_to_outputs.argtypes = [_p, _p, _p, ctypes.c_int32, _p, ctypes.c_int32, _d, ctypes.c_int32,
                                          _p, _p, _p, _p, _p, _p]
         _lib = L
@@ -608,6 +609,13 @@ class Batch:
 
     def run_to_outputs(self, graph=True, max_cycles=100000):
         self._chk(lib().gev_batch_run_to_outputs(self._b, int(graph), max_cycles), "batch_run_to_outputs")
+
+    def autotune(self):
+        """The tile autotuner's picks: {"y_row_bytes", "z_row_bytes", "pass_ms"}."""
+        y, z = ctypes.c_int32(), ctypes.c_int32()
+        ms = np.zeros((2, 3), dtype=np.float32)
+        self._chk(lib().gev_batch_autotune(self._b, ctypes.byref(y), ctypes.byref(z), _ptr(ms)), "batch_autotune")
+        return {"y_row_bytes": y.value, "z_row_bytes": z.value, "pass_ms": ms.tolist()}
 
     def synchronize(self):
         """Waits for the batch; checks the device time stepping against the host's."""

@@ -471,6 +471,14 @@ int gev_batch_step_outputs(gev_batch *batch, int32_t use_graph, int32_t *finishe
 int gev_batch_synchronize(gev_batch *batch);
 /* Steps until every running box has written its last output. */
 int gev_batch_run_to_outputs(gev_batch *batch, int32_t use_graph, int32_t max_cycles);
+/* The batch's tile autotuner (runs once, before the first cycle, if not
+   called): times the strided FFT passes' tiles with 64-, 128- and 256-byte
+   rows on the batch's FFT scratch over every box and keeps the fastest per
+   axis (N <= 256; results do not depend on the pick).  Returns the picks
+   (0: the default tile) and pass_ms[2][3], the candidates' times in ms per
+   launch (y then z; all zero when the pick came from an earlier batch of the
+   same grid, form and box count).  Any pointer may be NULL. */
+int gev_batch_autotune(gev_batch *batch, int32_t *row_bytes_y, int32_t *row_bytes_z, float *pass_ms);
 /* The [num_pk][numbins] tables of a finished box (see gev_run_to_outputs). */
 int gev_batch_get_outputs(gev_batch *batch, int32_t b, double *k, double *pk, double *sigma_k, double *sigma_pk,
                           int32_t *occupation, gev_pk_output_info *info);
