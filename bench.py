@@ -17,7 +17,10 @@ is timed once and reported beside the cycle throughput ("ic_generator").
 +-0.05-cell jitter and small random momenta (the CPU baseline's sample).
 
 Output: one JSON line.  `value` is particle-steps/s over the timed cycles
-(z = 100 down to about z = 60) with everything resident in HBM; `roofline`
+with everything resident in HBM.  The timed cycles start from the initial
+conditions at z = 100 (the warm-up cycles run first and the initial
+conditions are then generated again); the default 117 cycles are the whole
+run to z = 0.  `roofline`
 is the kernel with the largest share of the timed region, its algorithmic
 bytes per launch divided by its mean duration (HIP events on the context
 stream); `cpu_baseline` is the CPU oracle (the restated reference
@@ -182,33 +185,92 @@ def sim_setup(gev, N, boxsize):
     return sim, st
 
 
-def cpu_baseline(N_sample, cycles, threads):
-    """Times the CPU oracle on a bounded sample: an N_sample^3 box with the
-    same cell size and settings, `cycles` timed cycles after one warm-up."""
+def host_info(threads):
+    """The host the CPU baseline ran on: cores seen, CPU model, threads used
+    and a STREAM-style triad bandwidth with those threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     orc.build()
     orc.set_threads(threads)
-    N = N_sample
-    pos, q, ids = synthetic_state(N, 7)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "threads": threads,
+            "stream_triad_gbs": round(orc.stream_triad_gbs(), 1)}
+
+
+def cpu_baseline(state, N, boxsize, threads):
+    """Times the CPU oracle (the FP64 C restatement of the reference
+    algorithm, OpenMP) on a bounded sample of the workload: the device's
+    generated initial state at N^3, cycle 0 untimed (it skips the phi
+    update), then cycle 1 timed, with main.cpp's BENCHMARK categories."""
+    import oracle as orc
+    orc.set_threads(threads)
     ob = orc.OracleBox(N, N ** 3)
-    ob.pos[:] = pos
-    ob.q[:] = q
-    ob.id[:] = ids
-    ob.box.pmass = 0.31 / N ** 3
-    del pos, q, ids
+    order = np.argsort(state["ids"])
+    ob.pos[:] = state["pos"][order]
+    ob.q[:] = state["q"][order]
+    ob.id[:] = state["ids"][order]
+    del order
+    ob.box.pmass = state["pmass"]
+    for name in ("phi", "chi", "Bi", "BiFT", "scalarFT"):
+        getattr(ob, name)[:] = state[name]
     ob.file()
-    boxsize = 4.0 * N
     sim = orc.Sim(N, 1, 0, 48.0, 0.04, 100.0, 1.5 * boxsize ** 2 / C_LIGHT ** 2, orc.make_cosmo())
     a0 = 1.0 / 101.0
     st = orc.State(a0, 0.0, orc.lib().orc_next_dtau(orc.ctypes.byref(sim), a0), 0.0, 0.0, 0.0, 0.0, 0)
-    ob.cycle(sim, st)  # warm-up (cycle 0 skips the phi update)
+    ob.cycle(sim, st)
+    orc.timers_reset()
     t0 = time.perf_counter()
-    for _ in range(cycles):
-        ob.cycle(sim, st)
+    ob.cycle(sim, st)
     dt = time.perf_counter() - t0
+    cats = orc.timers()
     ob.close()
-    return N ** 3 * cycles / dt, dt
+    return N ** 3 / dt, dt, {k: round(v[0], 3) for k, v in cats.items() if k != "spectra_output"}
+
+
+def cpu_c1_full(threads):
+    """The plain CPU configuration in full on the oracle: GR, 64^3 grid and
+    particles, box 320 Mpc/h, BBKS LCDM ICs from seed 42, z = 100 to 0 with
+    the delta spectrum at z = 0; particle-steps/s over the main loop and the
+    BENCHMARK categories."""
+    import bbks
+    import oracle as orc
+    orc.set_threads(threads)
+    N, L, h = 64, 320.0, 0.67
+    k, d, t = bbks.bbks_table(np.logspace(-5.0, np.log10(300.0), 400), 100.0, 0.31, h)
+    sim = orc.Sim(N, 1, 0, 48.0, 0.04, 100.0, 1.5 * L * L / C_LIGHT ** 2, orc.make_cosmo())
+    ob = orc.OracleBox(N, N ** 3)
+    tpl = orc.load_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
+    ob.generate_ic(sim, orc.IC(42, 1, 1, N // 4, 1, 100.0, 2.1e-9, 0.965, 0.05, L), tpl,
+                   k * L, d, t * L / h, d, t * L / h)
+    a0 = 1.0 / 101.0
+    st = orc.State(a0, orc.particle_horizon(a0, sim.fourpiG, sim.cosmo),
+                   orc.lib().orc_next_dtau(orc.ctypes.byref(sim), a0), 0.0, 0.0, 0.0, 0.0, 0)
+    orc.timers_reset()
+    t0 = time.perf_counter()
+    ob.run_to_outputs(sim, st, [0.0], N // 2, L)
+    dt = time.perf_counter() - t0
+    cats = orc.timers()
+    ob.close()
+    return {"config": "GR 64^3, box 320 Mpc/h, BBKS seed 42, z = 100 -> 0, P(k) at z = 0",
+            "cycles": st.cycle, "main_loop_s": round(dt, 3), "value": round(N ** 3 * st.cycle / dt, 1),
+            "unit": "particle-steps/s", "categories_s": {k: round(v[0], 3) for k, v in cats.items()}}
+
+
+def download_state(gev, box):
+    """The box's carried state on the host (the CPU baseline's sample)."""
+    pos, q, ids = box.download_particles()
+    st = {"pos": pos, "q": q, "ids": ids, "pmass": 0.31 / len(ids)}
+    for f, name in ((gev.PHI, "phi"), (gev.CHI, "chi"), (gev.BI, "Bi"), (gev.BIFT, "BiFT"), (gev.SCALARFT, "scalarFT")):
+        st[name] = box.download_field(f)
+    return st
 
 
 def kernel_breakdown(kernels, steps, N, Np, f):
@@ -254,7 +316,7 @@ def precision_form_report(gev, box, st, box32, st32, sim, args, ms64):
     and states its error against the FP64 form: phi (max norm) and the
     matter power spectrum below k_Nyq/2."""
     N = box.N
-    el, k32 = timed_cycles(box32, sim, st32, args.warmup, args.steps, not args.no_timers)
+    el, k32 = timed_cycles(box32, sim, st32, 0, args.steps, not args.no_timers)
     ms32 = 1e3 * el / args.steps
     assert st32.a == st.a
     phi64 = box.download_field(gev.PHI)
@@ -273,7 +335,7 @@ def precision_form_report(gev, box, st, box32, st32, sim, args, ms64):
             "ms_per_step": round(ms32, 3), "value": round(box.N ** 3 * args.steps / el, 1),
             "speedup_vs_fp64": round(ms64 / ms32, 3),
             "phi_max_rel_err": phi_err, "pk_max_rel_err_below_half_nyquist": pk_err,
-            "pk_bins_compared": int(low.sum()), "cycles_from_same_state": args.warmup + args.steps,
+            "pk_bins_compared": int(low.sum()), "cycles_from_same_state": args.steps,
             "z_at_comparison": round(1.0 / st.a - 1.0, 4),
             "step_bytes": stage_bytes(N, N ** 3, 4),
             "kernels": kernel_breakdown(k32, args.steps, N, N ** 3, 4) if k32 else {}}
@@ -449,13 +511,11 @@ def ensemble_report(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=117, help="timed cycles from z = 100 (117: the whole run to z = 0)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ngrid", type=int, default=512)
     ap.add_argument("--boxsize", type=float, default=None, help="Mpc/h (default 4 * ngrid)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-ngrid", type=int, default=256)
-    ap.add_argument("--cpu-cycles", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-timers", action="store_true", help="time without per-kernel events")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_gr_cycle_512_fp64.txt"),
@@ -518,14 +578,40 @@ def main():
     t_setup = time.perf_counter() - t_setup
     print("setup %.1f s (N=%d, box %.0f Mpc/h)" % (t_setup, N, boxsize), file=sys.stderr, flush=True)
 
+    # the CPU baseline's sample: this initial state, on the host
+    cpu_state = None
+    if not args.no_cpu_baseline and args.state == "ic":
+        cpu_state = download_state(gev, box)
+
     # the FP32 precision form starts from the same state (gev_copy_state)
     box32 = st32 = None
     if not args.no_precision_form:
         box32 = gev.Box(N, Np, form=gev.FORM_FP32)
-        box32.copy_state_from(box)
-        st32 = gev.CycleState(st.a, st.tau, st.dtau, st.dtau_old, st.maxvel, st.T00hom, st.phi_k0, st.cycle, 0)
 
-    elapsed, kernels = timed_cycles(box, sim, st, args.warmup, args.steps, not args.no_timers)
+    def initial_state():
+        if args.state == "ic":
+            s0 = ic_state(gev, box, N, boxsize, args.seed)[1]
+        else:
+            p0, q0, i0 = synthetic_state(N, args.seed)
+            box.upload_particles(p0, q0, i0, 0.31 / Np)
+            del p0, q0, i0
+            s0 = sim_setup(gev, N, boxsize)[1]
+        s32 = None
+        if box32 is not None:
+            box32.copy_state_from(box)
+            s32 = gev.CycleState(s0.a, s0.tau, s0.dtau, s0.dtau_old, s0.maxvel, s0.T00hom, s0.phi_k0, s0.cycle, 0)
+        return s0, s32
+
+    # warm-up cycles (first use of every kernel), then the initial state again
+    if args.warmup > 0:
+        st, st32 = initial_state()
+        box.run_cycles(sim, st, args.warmup)
+        if box32 is not None:
+            box32.run_cycles(sim, st32, args.warmup)
+        box.synchronize()
+    st, st32 = initial_state()
+
+    elapsed, kernels = timed_cycles(box, sim, st, 0, args.steps, not args.no_timers)
     print("timed %d cycles in %.3f s; a = %.5f, T00hom = %.6f, maxvel = %.3e"
           % (args.steps, elapsed, st.a, st.T00hom, st.maxvel), file=sys.stderr, flush=True)
 
@@ -547,14 +633,22 @@ def main():
                     "algorithmic_bytes": b, "ms_per_launch": round(avg_s * 1e3, 4)}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if cpu_state is not None:
+        # the box's share of the host: 16 threads (the GPU host's CPU share per GPU)
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        v, dt = cpu_baseline(args.cpu_ngrid, args.cpu_cycles, threads)
+        host = host_info(threads)
+        print("cpu baseline: %d threads of %s, triad %.1f GB/s" % (threads, host["cpu_model"],
+              host["stream_triad_gbs"]), file=sys.stderr, flush=True)
+        v, dt, cats = cpu_baseline(cpu_state, N, boxsize, threads)
+        del cpu_state
+        c1 = cpu_c1_full(threads)
+        print("cpu baseline: %d^3 cycle %.1f s; C1 in full %.1f s" % (N, dt, c1["main_loop_s"]),
+              file=sys.stderr, flush=True)
         cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": "CPU oracle (FP64 C restatement of the reference cycle, OpenMP) on a %d^3 grid "
-                         "with %d^3 particles, box %.0f Mpc/h (same cell size and settings), %d timed "
-                         "GR cycles after 1 warm-up, %.1f s" % (args.cpu_ngrid, args.cpu_ngrid,
-                                                                4.0 * args.cpu_ngrid, args.cpu_cycles, dt)}
+               "sample": "CPU oracle (FP64 C restatement of the reference cycle, OpenMP, %d threads) on the same "
+                         "%d^3 workload from the device's generated initial state: cycle 1 (a full GR cycle, after "
+                         "cycle 0 untimed), %.1f s" % (threads, N, dt),
+               "categories_s": cats, "host": host, "c1_full_run": c1}
 
     out = {
         "metric": METRIC,
@@ -573,7 +667,8 @@ def main():
         "config": {"workload": "GR PM cycle (main.cpp:372-879), %d^3 grid, %d^3 CDM particles, box %.0f Mpc/h, "
                                "LCDM h=0.67 Om=0.31, Cf 48, steplimit 0.04, parabolic B, FP64 plain form"
                                % (N, N, boxsize),
-                   "ngrid": N, "particles": Np, "boxsize_mpc_h": boxsize, "precision": "fp64"},
+                   "ngrid": N, "particles": Np, "boxsize_mpc_h": boxsize, "precision": "fp64",
+                   "timed_cycles": {"count": args.steps, "from_z": 100.0, "to_z": round(1.0 / st.a - 1.0, 4)}},
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
                           "achieved": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",

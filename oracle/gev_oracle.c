@@ -29,6 +29,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -38,6 +39,68 @@
 #endif
 
 typedef struct { double re, im; } cplx;
+
+/* ---- wall-clock timers of main.cpp's BENCHMARK categories (main.cpp:907-922) */
+static double now_s(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
+
+static double g_timer[ORC_NTIMERS];
+static int64_t g_count[ORC_NTIMERS];
+
+void orc_timers_reset(void)
+{
+    memset(g_timer, 0, sizeof g_timer);
+    memset(g_count, 0, sizeof g_count);
+}
+
+void orc_timers_get(double *seconds, int64_t *counts)
+{
+    memcpy(seconds, g_timer, sizeof g_timer);
+    memcpy(counts, g_count, sizeof g_count);
+}
+
+void orc_timer_add(int which, double seconds)
+{
+    g_timer[which] += seconds;
+    g_count[which]++;
+}
+
+/* STREAM-style triad a = b + s c over three arrays of n doubles with the
+   current OpenMP threads; the best of `reps` runs in GB/s (24 n bytes). */
+double orc_stream_triad(int64_t n, int reps)
+{
+    double *a = malloc(sizeof(double) * n), *b = malloc(sizeof(double) * n), *c = malloc(sizeof(double) * n);
+    if (!a || !b || !c) {
+        free(a); free(b); free(c);
+        return -1.;
+    }
+    int64_t i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) {
+        a[i] = 0.;
+        b[i] = 1.;
+        c[i] = 2.;
+    }
+    double best = 0.;
+    for (int r = 0; r < reps; r++) {
+        const double t0 = now_s();
+#pragma omp parallel for schedule(static)
+        for (i = 0; i < n; i++) a[i] = b[i] + 3. * c[i];
+        const double dt = now_s() - t0;
+        const double gbs = 24. * (double)n / dt / 1e9;
+        if (gbs > best) best = gbs;
+    }
+    volatile double sink = a[n / 2];
+    (void)sink;
+    free(a); free(b); free(c);
+    return best;
+}
 
 static inline int wrapi(int i, int N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
 static inline int64_t lin(int N, int x, int y, int z)
@@ -1204,6 +1267,7 @@ void orc_cycle_fields(const orc_sim *sim, orc_box *b, orc_state *st)
     const double fourpiG = sim->fourpiG;
     const orc_cosmo *cs = &sim->cosmo;
     const double a = st->a;
+    double t0 = now_s(), tf;
 
     /* projections, main.cpp:378-450 */
     memset(b->source, 0, sizeof(double) * n3);
@@ -1217,6 +1281,17 @@ void orc_cycle_fields(const orc_sim *sim, orc_box *b, orc_state *st)
     }
     memset(b->Sij, 0, sizeof(double) * 6 * n3);
     orc_projection_Tij_project(N, b->n, b->pos, b->q, b->start, b->perm, b->pmass, b->Sij, a, b->phi, 1.);
+    orc_timer_add(ORC_T_PROJECTION, now_s() - t0);
+    t0 = now_s();
+    double t_fft = 0.;
+    int n_fft = 0;
+#define ORC_FFT(call)      \
+    do {                   \
+        tf = now_s();      \
+        call;              \
+        t_fft += now_s() - tf; \
+        n_fft++;           \
+    } while (0)
 
     if (sim->gr_flag > 0) {
         /* main.cpp:457-491 */
@@ -1226,34 +1301,38 @@ void orc_cycle_fields(const orc_sim *sim, orc_box *b, orc_state *st)
             orc_prepare_ft_source_scalar(N, b->phi, b->chi, b->source, cs->Omega_cdm + cs->Omega_b + 0.0,
                                          b->source, 3. * H * dx * dx / st->dtau_old,
                                          fourpiG * dx * dx / a, 3. * H * H * dx * dx);
-            orc_fft_r2c(N, b->source, b->scalarFT);
+            ORC_FFT(orc_fft_r2c(N, b->source, b->scalarFT));
             orc_solve_modified_poisson_ft(N, b->scalarFT, b->scalarFT, 1. / (dx * dx),
                                           3. * orc_hconf(a, fourpiG, cs) / st->dtau_old);
-            orc_fft_c2r(N, b->scalarFT, b->phi);
+            ORC_FFT(orc_fft_c2r(N, b->scalarFT, b->phi));
         }
     } else {
         /* main.cpp:493-516 */
-        orc_fft_r2c(N, b->source, b->scalarFT);
+        ORC_FFT(orc_fft_r2c(N, b->source, b->scalarFT));
         orc_solve_modified_poisson_ft(N, b->scalarFT, b->scalarFT, fourpiG / a, 0.);
-        orc_fft_c2r(N, b->scalarFT, b->phi);
+        ORC_FFT(orc_fft_c2r(N, b->scalarFT, b->phi));
     }
     st->phi_k0 = b->scalarFT[0]; /* main.cpp:533 */
 
     /* main.cpp:539-598 */
     orc_prepare_ft_source_tensor(N, b->phi, b->Sij, b->Sij, 2. * fourpiG * dx * dx / a);
     for (int c = 0; c < 6; c++)
-        orc_fft_r2c(N, b->Sij + c * n3, b->SijFT + (size_t)c * N * N * (N / 2 + 1) * 2);
+        ORC_FFT(orc_fft_r2c(N, b->Sij + c * n3, b->SijFT + (size_t)c * N * N * (N / 2 + 1) * 2));
     orc_project_ft_scalar(N, b->SijFT, b->scalarFT, 0);
-    orc_fft_c2r(N, b->scalarFT, b->chi);
+    ORC_FFT(orc_fft_c2r(N, b->scalarFT, b->chi));
     const size_t nk2 = (size_t)N * N * (N / 2 + 1) * 2;
     if (sim->vector_flag == 1) {
-        for (int c = 0; c < 3; c++) orc_fft_r2c(N, b->Bi + c * n3, b->BiFT + c * nk2);
+        for (int c = 0; c < 3; c++) ORC_FFT(orc_fft_r2c(N, b->Bi + c * n3, b->BiFT + c * nk2));
         orc_project_ft_vector(N, b->BiFT, b->BiFT, fourpiG * dx * dx, 0.);
     } else {
         orc_evolve_ft_vector(N, b->SijFT, b->BiFT, a * a * st->dtau_old);
     }
     if (sim->gr_flag > 0)
-        for (int c = 0; c < 3; c++) orc_fft_c2r(N, b->BiFT + c * nk2, b->Bi + c * n3);
+        for (int c = 0; c < 3; c++) ORC_FFT(orc_fft_c2r(N, b->BiFT + c * nk2, b->Bi + c * n3));
+#undef ORC_FFT
+    orc_timer_add(ORC_T_GRAVITY, now_s() - t0);
+    g_timer[ORC_T_FFT] += t_fft;
+    g_count[ORC_T_FFT] += n_fft;
 }
 
 /* The particle stage of the cycle (main.cpp:766-822, 867-874): kick, half
@@ -1265,6 +1344,7 @@ void orc_cycle_particles(const orc_sim *sim, orc_box *b, orc_state *st)
     const orc_cosmo *cs = &sim->cosmo;
     double a = st->a;
     double params[2];
+    double t0 = now_s();
 
     params[0] = a;
     params[1] = a * a * N;
@@ -1274,6 +1354,8 @@ void orc_cycle_particles(const orc_sim *sim, orc_box *b, orc_state *st)
     else
         st->maxvel = orc_update_vel(N, b->n, b->pos, b->q, ORC_Q_NEWTON, (st->dtau + st->dtau_old) / 2.,
                                     b->phi, b->chi, b->Bi, 1, params);
+    orc_timer_add(ORC_T_UPDATE_Q, now_s() - t0);
+    t0 = now_s();
     orc_rungekutta4bg(&a, fourpiG, cs, 0.5 * st->dtau);
     params[0] = a;
     params[1] = a * a * N;
@@ -1285,6 +1367,7 @@ void orc_cycle_particles(const orc_sim *sim, orc_box *b, orc_state *st)
     orc_rungekutta4bg(&a, fourpiG, cs, 0.5 * st->dtau);
     if (sim->gr_flag > 0) st->maxvel /= sqrt(st->maxvel * st->maxvel + 1.0);
     orc_file_particles(N, b->n, b->pos, b->id, b->cell, b->count, b->start, b->perm);
+    orc_timer_add(ORC_T_MOVE, now_s() - t0);
 
     st->tau += st->dtau;
     st->dtau_old = st->dtau;

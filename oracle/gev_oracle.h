@@ -110,6 +110,16 @@ double orc_field_sum(int64_t count, const double *f);
 /* OpenMP thread count of subsequent calls (tests: results must not depend on it). */
 void orc_set_threads(int n);
 
+/* Wall-clock time of the cycle's parts, in main.cpp's BENCHMARK categories
+   (main.cpp:907-922): projections, gravity solver (its FFTs also counted
+   apart), update momenta, move particles, spectra output; counts of calls. */
+enum { ORC_T_PROJECTION = 0, ORC_T_GRAVITY, ORC_T_FFT, ORC_T_UPDATE_Q, ORC_T_MOVE, ORC_T_SPECTRA, ORC_NTIMERS };
+void orc_timers_reset(void);
+void orc_timers_get(double *seconds, int64_t *counts);
+void orc_timer_add(int which, double seconds);
+/* STREAM-style triad bandwidth of the host (GB/s, best of reps). */
+double orc_stream_triad(int64_t n, int reps);
+
 /* ---- one cycle of main.cpp:372-879 (CDM only, no outputs) ---------------- */
 typedef struct {
     int N;

@@ -14,6 +14,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -119,6 +123,15 @@ void orc_extract_cross_spectrum(int N, const double *f1, const double *f2, int n
 /* the main loop with power-spectrum outputs                                 */
 /* ======================================================================== */
 
+static double omp_wtime_or_clock(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
+
 /* A number as writePowerSpectrum prints it ("%e" for the table, "%f" for the
    header redshift) and the interpolating pass reads it back (fscanf %le /
    %lf), tools.hpp:283-301,318-332. */
@@ -163,7 +176,9 @@ static void write_spectrum(const orc_sim *sim, orc_box *b, const orc_state *st, 
 {
     const int nb = w->numbins;
     double aw = st->a;
+    const double t0 = omp_wtime_or_clock();
     delta_spectrum(sim, b, st->a, nb, w->kb, w->pw, w->ks, w->ps, w->oc);
+    orc_timer_add(ORC_T_SPECTRA, omp_wtime_or_clock() - t0);
     for (int i = 0; i < nb; i++) w->p2[i] = w->pw[i] / w->rescalep;
     if (1. / aw < zt + 1.) {
         if (w->have[j]) { /* the file of the look-ahead write exists: interpolate */

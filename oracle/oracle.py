@@ -69,6 +69,10 @@ def lib():
         L.orc_run_to_outputs.restype = _i
         L.orc_run_to_outputs.argtypes = [_p, _p, _p, _i, _p, _i, _d, _i, _p, _p, _p, _p, _p, _p, _p]
         L.orc_set_threads.argtypes = [_i]
+        L.orc_timers_reset.argtypes = []
+        L.orc_timers_get.argtypes = [_p, _p]
+        L.orc_stream_triad.restype = _d
+        L.orc_stream_triad.argtypes = [_i64, _i]
         _u32, _u64 = ctypes.c_uint32, ctypes.c_uint64
         L.orc_prng_block.argtypes = [_u32, _u64, _p]
         L.orc_prng_at.restype = _u32
@@ -135,6 +139,25 @@ def make_cosmo(Omega_cdm=0.26, Omega_b=0.05, h=0.67, Omega_rad=0.0):
 
 
 # ---------------------------------------------------------------- wrappers
+
+TIMER_NAMES = ("projections", "gravity_solver", "fft", "update_momenta", "move_particles", "spectra_output")
+
+
+def timers_reset():
+    lib().orc_timers_reset()
+
+
+def timers():
+    """{category: (seconds, calls)} since the last reset (main.cpp:907-922 categories)."""
+    sec = np.zeros(len(TIMER_NAMES))
+    cnt = np.zeros(len(TIMER_NAMES), dtype=np.int64)
+    lib().orc_timers_get(_ptr(sec), _ptr(cnt))
+    return {n: (float(s), int(c)) for n, s, c in zip(TIMER_NAMES, sec, cnt)}
+
+
+def stream_triad_gbs(n=1 << 26, reps=5):
+    return lib().orc_stream_triad(n, reps)
+
 
 def set_threads(n):
     lib().orc_set_threads(n)
