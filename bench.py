@@ -392,6 +392,64 @@ def full_run_report(gev, box, box32, N, boxsize, seed):
     return out
 
 
+def c3_report(gev):
+    """The precision form at 1024^3 (box 4096 Mpc/h, BBKS LCDM ICs from seed
+    11, GR): the FP64 form (compact: 277 GB, one particle set) and the FP32
+    form (compact, 177 GB) from the same generated initial state, each run
+    from z = 100 through the delta spectrum outputs at z = 10, 1 and 0
+    (gev_run_to_outputs).  The two boxes do not fit the GPU together, so the
+    state goes through host memory.  The FP32 form's phi (max norm) and P(k)
+    (bins below k_Nyq/2) errors against the FP64 form at each output."""
+    N, L, seed = 1024, 4096.0, 11
+    Np = N ** 3
+    zs = (10.0, 1.0, 0.0)
+    box = gev.Box(N, Np, compact=True)
+    sim, st, ic_ms, _, _ = ic_state(gev, box, N, L, seed)
+    state = download_state(gev, box)
+    print("c3: 1024^3 ICs in %.0f ms, state on the host" % ic_ms, file=sys.stderr, flush=True)
+    ref = []
+    t64 = 0.0
+    for zt in zs:
+        box.synchronize()
+        t0 = time.perf_counter()
+        o = box.run_to_outputs(sim, st, [zt], N // 2, L)
+        t64 += time.perf_counter() - t0
+        ref.append((st.cycle, o, box.download_field(gev.PHI)))
+        print("c3: FP64 at z = %g after %d cycles, %.1f s" % (zt, st.cycle, t64), file=sys.stderr, flush=True)
+    cyc = st.cycle
+    box.close()
+    b32 = gev.Box(N, Np, form=gev.FORM_FP32, compact=True)
+    b32.upload_particles(state["pos"], state["q"], state["ids"], state["pmass"])
+    for f, name in ((gev.PHI, "phi"), (gev.CHI, "chi"), (gev.BI, "Bi"), (gev.BIFT, "BiFT"), (gev.SCALARFT, "scalarFT")):
+        b32.upload_field(f, state[name])
+    del state
+    st32 = sim_setup(gev, N, L)[1]
+    t32, points = 0.0, []
+    for zt, (c64, o64, phi64) in zip(zs, ref):
+        b32.synchronize()
+        t0 = time.perf_counter()
+        o32 = b32.run_to_outputs(sim, st32, [zt], N // 2, L)
+        t32 += time.perf_counter() - t0
+        assert st32.cycle == c64
+        phi32 = b32.download_field(gev.PHI)
+        low = (o64["occupation"][0] > 0) & (o64["k"][0] < 0.5 * np.pi * N / L)
+        points.append({"z": zt, "cycle": c64,
+                       "phi_max_rel_err": float(np.max(np.abs(phi32 - phi64)) / np.max(np.abs(phi64))),
+                       "pk_max_rel_err_below_half_nyquist": float(np.max(np.abs(o32["pk"][0][low] - o64["pk"][0][low])
+                                                                         / np.abs(o64["pk"][0][low])))})
+        del phi32
+        print("c3: FP32 at z = %g, phi err %.2e, P(k) err %.2e" % (zt, points[-1]["phi_max_rel_err"],
+              points[-1]["pk_max_rel_err_below_half_nyquist"]), file=sys.stderr, flush=True)
+    b32.close()
+    return {"config": "GR 1024^3 grid and particles, box 4096 Mpc/h, BBKS LCDM ICs seed 11, z = 100 -> 0",
+            "boxes": "compact (one particle set): FP64 %.1f GB, FP32 %.1f GB"
+                     % (gev.footprint_bytes(N, Np, gev.FORM_FP64, True) / 1e9,
+                        gev.footprint_bytes(N, Np, gev.FORM_FP32, True) / 1e9),
+            "cycles": cyc, "fp64_time_to_z0_s": round(t64, 2), "fp32_time_to_z0_s": round(t32, 2),
+            "fp64_value": round(Np * cyc / t64, 1), "fp32_value": round(Np * cyc / t32, 1),
+            "speedup_vs_fp64": round(t64 / t32, 3), "checkpoints": points}
+
+
 def serving_report(args):
     """Whole simulations served on one GPU (serving.py) against back-to-back
     plain FP64 runs on the same Poisson arrival trace: request grids from
@@ -532,6 +590,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--serving-rates", default="12,48", help="Poisson arrival rates, requests/s")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble report")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 1024^3 precision-form comparison")
     ap.add_argument("--ensemble-boxes", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 1 GiB device copies (scale for the PMC byte counters)")
@@ -686,6 +745,8 @@ def main():
     if box32 is not None:
         box32.close()
     box.close()
+    if not args.no_c3 and out["precision_form"] is not None:
+        out["precision_form"]["c3_1024"] = c3_report(gev)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_report(args)
     if not args.no_serving:

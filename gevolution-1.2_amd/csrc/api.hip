@@ -104,9 +104,7 @@ static Sizes sizes_of(int N, int64_t cap) {
     return s;
 }
 
-int64_t slab_bytes(int N, int64_t cap, bool fp32);
-
-static int64_t footprint(int N, int64_t cap, bool fp32) { return slab_bytes(N, cap, fp32); }
+int64_t slab_bytes(int N, int64_t cap, bool fp32, bool compact);
 
 static void free_all(gev_ctx *c) {
     if (c->view) return;  // a view owns nothing
@@ -138,7 +136,8 @@ using namespace gev;
 GEV_EXPORT int64_t gev_ctx_footprint_bytes(const gev_box_desc *d) {
     if (!d || d->ngrid < 2 || d->npart < 0) return -1;
     if (d->form != GEV_FORM_FP64 && d->form != GEV_FORM_FP32) return -1;
-    return footprint(d->ngrid, d->npart, d->form == GEV_FORM_FP32);
+    if ((d->flags & ~GEV_BOX_COMPACT) != 0) return -1;
+    return slab_bytes(d->ngrid, d->npart, d->form == GEV_FORM_FP32, (d->flags & GEV_BOX_COMPACT) != 0);
 }
 
 namespace gev {
@@ -165,6 +164,14 @@ struct SlabLayout {
     }
 };
 
+// a compact box's second particle set is its only one
+static void alias_compact(gev_ctx *c) {
+    if (!c->compact) return;
+    c->pos2 = c->pos;
+    c->q2 = c->q;
+    c->id2 = c->id;
+}
+
 static void slab_layout(gev_ctx *c, const Sizes &s, SlabLayout &L) {
     const int64_t rb = rbytes(c), cb = cbytes(c);
     L.add((void **)&c->phi, s.n3 * rb);
@@ -183,9 +190,11 @@ static void slab_layout(gev_ctx *c, const Sizes &s, SlabLayout &L) {
     L.add((void **)&c->pos, 3 * s.cap * 8);
     L.add((void **)&c->q, 3 * s.cap * 8);
     L.add((void **)&c->id, s.cap * 8);
-    L.add((void **)&c->pos2, 3 * s.cap * 8);
-    L.add((void **)&c->q2, 3 * s.cap * 8);
-    L.add((void **)&c->id2, s.cap * 8);
+    if (!c->compact) {
+        L.add((void **)&c->pos2, 3 * s.cap * 8);
+        L.add((void **)&c->q2, 3 * s.cap * 8);
+        L.add((void **)&c->id2, s.cap * 8);
+    }
     L.add((void **)&c->cell, s.cap * 4);
     L.add((void **)&c->perm, s.cap * 4);
     L.add((void **)&c->count, s.n3 * 4);
@@ -205,13 +214,15 @@ void assign_box_pointers(gev_ctx *c, char *base) {
     SlabLayout L;
     slab_layout(c, sizes_of(c->N, c->cap), L);
     L.assign(base);
+    alias_compact(c);
 }
 
 // bytes of one box's slab
-int64_t slab_bytes(int N, int64_t cap, bool fp32) {
+int64_t slab_bytes(int N, int64_t cap, bool fp32, bool compact) {
     gev_ctx c;
     c.N = N;
     c.fp32 = fp32;
+    c.compact = compact;
     SlabLayout L;
     slab_layout(&c, sizes_of(N, cap), L);
     return L.size();
@@ -223,12 +234,14 @@ int create_ctx(const gev_box_desc *d, int nbox, gev_ctx **out) {
     if (!out) return GEV_EINVAL;
     *out = nullptr;
     if (!d || d->npart < 0 || d->npart > 0xFFFFFFFFll || nbox < 1) return GEV_EINVAL;
+    if ((d->flags & ~GEV_BOX_COMPACT) != 0 || ((d->flags & GEV_BOX_COMPACT) && nbox > 1)) return GEV_EINVAL;
     if (d->form != GEV_FORM_FP64 && d->form != GEV_FORM_FP32) return GEV_EINVAL;
     if (!fft_size_supported(d->ngrid)) return GEV_EINVAL;
     gev_ctx *c = new (std::nothrow) gev_ctx();
     if (!c) return GEV_ENOMEM;
     c->N = d->ngrid;
     c->fp32 = d->form == GEV_FORM_FP32;
+    c->compact = (d->flags & GEV_BOX_COMPACT) != 0;
     c->P = kpitch(c->N);
     c->cap = d->npart;
     (void)hipGetDevice(&c->device);
@@ -248,7 +261,10 @@ int create_ctx(const gev_box_desc *d, int nbox, gev_ctx **out) {
     c->stride = L.size();
     bool ok = hipMalloc(&c->slab, (size_t)c->stride * nbox) == hipSuccess;
     if (!ok) c->slab = nullptr;
-    if (ok) L.assign((char *)c->slab);
+    if (ok) {
+        L.assign((char *)c->slab);
+        alias_compact(c);
+    }
     if (ok && hipHostMalloc((void **)&c->host_pinned, 64) != hipSuccess) ok = false;
     if (ok && hipHostMalloc((void **)&c->rec_pinned, sizeof(double) * 2 * REC_SLOTS) != hipSuccess) ok = false;
     if (!ok) {

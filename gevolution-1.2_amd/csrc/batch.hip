@@ -548,7 +548,7 @@ GEV_EXPORT int gev_batch_create(const gev_batch_desc *d, gev_batch **out) {
     if (!d || d->nbox < 1 || d->gr_flag < 0 || d->gr_flag > 1) return GEV_EINVAL;
     gev_batch *B = new (std::nothrow) gev_batch();
     if (!B) return GEV_ENOMEM;
-    gev_box_desc bd{d->ngrid, d->form, d->npart, d->stream};
+    gev_box_desc bd{d->ngrid, d->form, d->npart, d->stream, 0, 0};
     int rc = create_ctx(&bd, d->nbox, &B->base);
     if (rc) {
         delete B;

@@ -120,6 +120,7 @@ struct gev_ctx {
 
     int N = 0;
     int P = 0;            // k-space row pitch (complex elements)
+    bool compact = false; // one particle set, drifts in place (GEV_BOX_COMPACT)
     int64_t cap = 0;      // particle capacity
     int64_t n = 0;        // particles resident
     double pmass = 0.0;   // species mass (part_simple_info)
@@ -316,7 +317,7 @@ int write_spectrum(gev_ctx *c, const gev_sim_params *p, const gev_cycle_state *s
 // batches (api.hip): a box's array pointers at its slab; a batch's contexts
 void assign_box_pointers(gev_ctx *c, char *base);
 int create_ctx(const gev_box_desc *d, int nbox, gev_ctx **out);
-int64_t slab_bytes(int N, int64_t cap, bool fp32);
+int64_t slab_bytes(int N, int64_t cap, bool fp32, bool compact = false);
 
 int launch_refile(gev_ctx *ctx);  // cells from positions, then CSR
 int launch_particles_batch(gev_ctx *ctx, bool gr, const Batch &bt);

@@ -843,9 +843,10 @@ struct StepArgs {
 #define GEV_KICK_OCC
 #endif
 
-// BATCH: the launch covers the boxes of a batch (its own instantiation, so
-// the one-box kernel keeps its register budget).
-template <bool KICK, int QM, bool DRIFT, int PM, bool POW2, typename F, bool BATCH = false>
+// BATCH: the launch covers the boxes of a batch; INPLACE: a compact box, the
+// drift writes each particle back at its own index (each its own
+// instantiation, so the plain one-box kernel keeps its register budget).
+template <bool KICK, int QM, bool DRIFT, int PM, bool POW2, typename F, bool BATCH = false, bool INPLACE = false>
 __global__ __launch_bounds__(TPB, (sizeof(F) == 4 && !BATCH ? GEV_KICK_WAVES_F32 : 1)) GEV_KICK_OCC void k_update_tiled(
     double *__restrict__ pos, double *__restrict__ q, int64_t cap, int N, int ntx, int nty, int ntz,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm, const F *__restrict__ phi,
@@ -1067,17 +1068,21 @@ __global__ __launch_bounds__(TPB, (sizeof(F) == 4 && !BATCH ? GEV_KICK_WAVES_F32
             }
 #pragma unroll
             for (int l = 0; l < 3; l++) x[l] = wrap_pos<POW2>(x[l], dx, N);
-            // stored at the sorted index: the storage order becomes the cell order
-            pos_out[js] = x[0];
-            pos_out[cap + js] = x[1];
-            pos_out[2 * cap + js] = x[2];
-            q_out[js] = qq[0];
-            q_out[cap + js] = qq[1];
-            q_out[2 * cap + js] = qq[2];
-            id_out[js] = pid;
+            // stored at the sorted index in the other set (the storage order
+            // becomes the cell order), or in place in a compact box
+            const uint32_t w = INPLACE ? p : js;
+            double *const po = INPLACE ? pos : pos_out;
+            double *const qo = INPLACE ? q : q_out;
+            po[w] = x[0];
+            po[cap + w] = x[1];
+            po[2 * cap + w] = x[2];
+            qo[w] = qq[0];
+            qo[cap + w] = qq[1];
+            qo[2 * cap + w] = qq[2];
+            if (!INPLACE) id_out[js] = pid;
             const uint32_t c = (uint32_t)L(N, cell1<POW2>(x[0], dx, N), cell1<POW2>(x[1], dx, N),
                                            cell1<POW2>(x[2], dx, N));
-            cell[js] = c;
+            cell[w] = c;
             atomicAdd(count + c, 1u);  // no return value: the wave does not wait for it
         }
         // the next particle was fetched one step ahead; fetch the one after
@@ -1120,21 +1125,26 @@ void launch_update_f(gev_ctx *ctx, const StepArgs &sa, const gev::Batch &bt) {
     const int64_t nblk = (int64_t)ntx * nty * ntz;
     const unsigned nbox = bt.cyc ? (unsigned)ctx->nbox : 1u;
     const F *phi = gev::fld<F>(ctx->phi), *chi = gev::fld<F>(ctx->chi), *Bi = gev::fld<F>(ctx->Bi);
-#define GEV_KICK_LAUNCH(P2, B)                                                                                 \
-    hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, P2, F, B>), dim3((unsigned)nblk, nbox), dim3(TPB), 0, \
-                       ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm, phi, chi,  \
+#define GEV_KICK_LAUNCH(P2, B, IP)                                                                             \
+    hipLaunchKernelGGL((k_update_tiled<KICK, QM, DRIFT, PM, P2, F, B, IP>), dim3((unsigned)nblk, nbox), dim3(TPB),  \
+                       0, ctx->stream, ctx->pos, ctx->q, ctx->cap, N, ntx, nty, ntz, ctx->start, ctx->perm, phi, chi, \
                        Bi, sa, ctx->red_u, ctx->count, ctx->cell, ctx->pos2, ctx->q2, ctx->id, ctx->id2, bt)
     const bool pow2 = (N & (N - 1)) == 0;
     if (bt.cyc) {
         // the batched cycle runs the kick and drift together only
         if constexpr (KICK && DRIFT) {
-            if (pow2) GEV_KICK_LAUNCH(true, true);
-            else GEV_KICK_LAUNCH(false, true);
+            if (pow2) GEV_KICK_LAUNCH(true, true, false);
+            else GEV_KICK_LAUNCH(false, true, false);
+        }
+    } else if (ctx->compact && DRIFT) {
+        if constexpr (DRIFT) {
+            if (pow2) GEV_KICK_LAUNCH(true, false, true);
+            else GEV_KICK_LAUNCH(false, false, true);
         }
     } else if (pow2) {
-        GEV_KICK_LAUNCH(true, false);
+        GEV_KICK_LAUNCH(true, false, false);
     } else {
-        GEV_KICK_LAUNCH(false, false);
+        GEV_KICK_LAUNCH(false, false, false);
     }
 #undef GEV_KICK_LAUNCH
 }
@@ -1275,7 +1285,7 @@ int launch_particle_update(gev_ctx *ctx, bool kick, int qm, double dtk, int nfk,
     }
     rc = check_hip(ctx, hipGetLastError(), "particle update launch");
     if (rc) return rc;
-    if (drift && ctx->n > 0) {
+    if (drift && ctx->n > 0 && !ctx->compact) {
         // the drift wrote the particles in cell order into the second set
         std::swap(ctx->pos, ctx->pos2);
         std::swap(ctx->q, ctx->q2);

@@ -55,13 +55,15 @@ EXPORTED = [
 ]
 
 
-# forms of a box (include/gev_pm.h)
+# forms and flags of a box (include/gev_pm.h)
 FORM_FP64, FORM_FP32 = 0, 1
+BOX_COMPACT = 1
 
 
 class BoxDesc(ctypes.Structure):
     _fields_ = [("ngrid", ctypes.c_int32), ("form", ctypes.c_int32),
-                ("npart", ctypes.c_int64), ("stream", _p)]
+                ("npart", ctypes.c_int64), ("stream", _p), ("flags", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32)]
 
 
 class Cosmology(ctypes.Structure):
@@ -229,8 +231,8 @@ def rungekutta4bg(a, fourpiG, cosmo, dtau):
     return x.value
 
 
-def footprint_bytes(ngrid, npart, form=FORM_FP64):
-    d = BoxDesc(ngrid, form, npart, None)
+def footprint_bytes(ngrid, npart, form=FORM_FP64, compact=False):
+    d = BoxDesc(ngrid, form, npart, None, BOX_COMPACT if compact else 0, 0)
     return lib().gev_ctx_footprint_bytes(ctypes.byref(d))
 
 
@@ -246,13 +248,13 @@ class Box:
         b._owned = False
         return b
 
-    def __init__(self, ngrid, npart, stream=None, form=FORM_FP64):
+    def __init__(self, ngrid, npart, stream=None, form=FORM_FP64, compact=False):
         self._owned = True
         self.N = int(ngrid)
         self.cap = int(npart)
         self.form = int(form)
         self._c = _p()
-        d = BoxDesc(self.N, self.form, self.cap, stream)
+        d = BoxDesc(self.N, self.form, self.cap, stream, BOX_COMPACT if compact else 0, 0)
         rc = lib().gev_ctx_create(ctypes.byref(d), ctypes.byref(self._c))
         if rc != 0:
             raise GevError("gev_ctx_create(N=%d, npart=%d, form=%d) failed with %d"

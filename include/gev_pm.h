@@ -82,11 +82,22 @@ enum {
                           denominators and zero mode */
 };
 
+/* flags of a box (gev_box_desc.flags) */
+enum {
+    GEV_BOX_COMPACT = 1 /* one particle set: a drift writes each particle back
+                           in place instead of storing the particles in cell
+                           order into a second set (56 bytes less per particle:
+                           a 1024^3 FP64 box fits one 288 GB MI355X; particle
+                           passes then read through the filing's permutation) */
+};
+
 typedef struct {
     int32_t ngrid;      /* N: lattice points per dimension (2^a 3^b, 8..1024)  */
     int32_t form;       /* GEV_FORM_FP64 (0) or GEV_FORM_FP32                  */
     int64_t npart;      /* particle capacity                                   */
     void *stream;       /* hipStream_t to run on; NULL = a new stream          */
+    int32_t flags;      /* GEV_BOX_* flags (0: none)                           */
+    int32_t reserved0;
 } gev_box_desc;
 
 /* ---- lifecycle ---------------------------------------------------------- */
