@@ -576,7 +576,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-timers", action="store_true", help="time without per-kernel events")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_gr_cycle_512_fp64.txt"),
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_gr_run_512_fp64.txt"),
                     help="measured HBM bytes per launch (rocprofv3 --pmc summary)")
     ap.add_argument("--state", choices=["ic", "lattice"], default="ic",
                     help="initial state: device IC generator (default) or the jittered lattice")

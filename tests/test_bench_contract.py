@@ -34,7 +34,7 @@ def test_step_bytes_match_the_stage_table():
 
 
 def test_measured_traffic_reads_the_committed_profile():
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_gr_cycle_512_fp64.txt")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_gr_run_512_fp64.txt")
     t = bench.measured_traffic(path, "projection_T00_Tij_project")
     assert t is not None and t > bench.algorithmic_bytes("projection_T00_Tij_project", N, NP)
     assert bench.measured_traffic(path, "no_such_kernel") is None
