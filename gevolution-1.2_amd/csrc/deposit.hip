@@ -17,6 +17,14 @@
 // Tiles whose source cells hold more particles than the LDS holds are
 // processed in chunks of whole source cells (still in site order); a single
 // cell larger than a chunk is read directly from global memory.
+//
+// Clustered chunks (FP64 T00 + Tij): a target thread walks all particles of
+// its 8 source cells, so one dense cell keeps its wavefront busy while the
+// other lanes idle.  The dense cells of such a chunk are summed first, one
+// lane per (cell, target) pair and 8 cells per wavefront, in the same
+// particle order; the partial sums replace the cell's records in LDS and the
+// target threads add them in site order as before, so the result does not
+// change by a bit.
 #include "gev_internal.h"
 
 namespace {
@@ -55,6 +63,34 @@ template <typename R> constexpr int dep_cap() { return sizeof(R) == 4 ? GEV_DEP_
 #ifndef GEV_DEP_FIXED_MIN
 #define GEV_DEP_FIXED_MIN 448
 #endif
+// Source cells with at least this many particles in a chunk are "dense"
+// (T00 + Tij deposit): their partial sums for all their in-tile targets are
+// computed first by lanes that each take one (cell, target) pair, and the
+// target threads read them instead of walking the cell's particles (0: off).
+#ifndef GEV_DEP_DENSE
+#define GEV_DEP_DENSE 4
+#endif
+// (the FP32 form: off; its dense tiles take the fixed-point path, and
+// without the dense phase its gather keeps 4 wavefronts per SIMD)
+#ifndef GEV_DEP_DENSE_F32
+#define GEV_DEP_DENSE_F32 0
+#endif
+template <typename R> constexpr int dense_min() { return sizeof(R) == 4 ? GEV_DEP_DENSE_F32 : GEV_DEP_DENSE; }
+// ... and the dense phase runs only in chunks with a cell of at least this
+// many particles (below that the target threads' walks are balanced enough)
+#ifndef GEV_DEP_DENSE_TRIG
+#define GEV_DEP_DENSE_TRIG 8
+#endif
+// the count of a cell whose partial sums the dense phase has formed carries
+// this bit
+constexpr uint32_t DENSE_MARK = 0x80000000u;
+// A dense cell's partial sums are stored over its own records (record
+// arrays 0..11, rows r0 .. r0+n-1): lc, ld0, ld1, ld2 of target offset d in
+// arrays 0..3 (d < 4) or 4..7 (d >= 4), row r0 + (d & 3); lo01 of the
+// offsets 0 and 4, lo02 of 0 and 2, lo12 of 0 and 1 in arrays 8, 9, 10, row
+// r0 + (which of the two).  Four records hold them all.
+static_assert(GEV_DEP_DENSE == 0 || GEV_DEP_DENSE >= 4, "a dense cell's records hold its partial sums");
+static_assert(GEV_DEP_DENSE_F32 == 0 || GEV_DEP_DENSE_F32 >= 4, "a dense cell's records hold its partial sums");
 static_assert(NSRC <= 2 * TPB, "two source cells per thread");
 static_assert(TPB % 64 == 0 && TPB <= 1024, "whole wavefronts");
 
@@ -236,6 +272,16 @@ template <int KIND, typename R> struct Stage {
             w12[i] = (R)r.w12;
         }
     }
+    // row r of record array k (the T00 + Tij stage: twelve record arrays of
+    // dep_cap<R>() values each, back to back)
+    __device__ __forceinline__ R &slot(int k, int r) {
+        static_assert(KIND == 3, "the T00 + Tij stage");
+        return (&up0[0])[k * dep_cap<R>() + r];
+    }
+    __device__ __forceinline__ R slot(int k, int r) const {
+        static_assert(KIND == 3, "the T00 + Tij stage");
+        return (&up0[0])[k * dep_cap<R>() + r];
+    }
     __device__ __forceinline__ Rec<R> get(int i) const {
         using K = Kind<KIND>;
         Rec<R> r;
@@ -338,8 +384,22 @@ __device__ __forceinline__ void add_source_cell(Acc<R> &A, const ST &st, const T
     Partial<R> P;
     P.clear();
     const uint32_t r0 = s_off[c] - base;
+    if (KIND == 3 && dense_min<R>() > 0 && (n & DENSE_MARK)) {
+        // a dense cell: its partial sums for this offset, already summed
+        if constexpr (KIND == 3) {
+            constexpr int d = DX | (DY << 1) | (DZ << 2), a0 = 4 * (d >> 2), row = d & 3;
+            P.lc = st.slot(a0, (int)r0 + row);
+            P.ld0 = st.slot(a0 + 1, (int)r0 + row);
+            P.ld1 = st.slot(a0 + 2, (int)r0 + row);
+            P.ld2 = st.slot(a0 + 3, (int)r0 + row);
+            if (DX == 0 && DY == 0) P.lo01 = st.slot(8, (int)r0 + DZ);
+            if (DX == 0 && DZ == 0) P.lo02 = st.slot(9, (int)r0 + DY);
+            if (DY == 0 && DZ == 0) P.lo12 = st.slot(10, (int)r0 + DX);
+        }
+    } else {
 #pragma unroll 1
-    for (uint32_t i = 0; i < n; i++) add_particle<KIND>(P, st.get((int)(r0 + i)), T, DX, DY, DZ);
+        for (uint32_t i = 0; i < n; i++) add_particle<KIND>(P, st.get((int)(r0 + i)), T, DX, DY, DZ);
+    }
     add_source<KIND>(A, P, T, DX, DY, DZ, mass_s);
 }
 
@@ -379,6 +439,88 @@ __device__ __forceinline__ void store_target(const Acc<F> &A, F *__restrict__ ou
     default: CALL(1, 1, 1); break; \
     }
 
+// the target-side values of cell (x, y, z): phi there, its plaquette sums
+// (gevolution.hpp:1263-1272) and link factors (1129-1144)
+template <bool USE_PHI, typename F>
+__device__ __forceinline__ Target<F> load_target(const F *__restrict__ phi, int N, int x, int y, int z) {
+    Target<F> T;
+    T.ph = (F)0;
+    F phx = 0, phy = 0, phz = 0, phxy = 0, phxz = 0, phyz = 0;
+    if (USE_PHI) {
+        const int xp = x == N - 1 ? 0 : x + 1, yp = y == N - 1 ? 0 : y + 1, zp = z == N - 1 ? 0 : z + 1;
+        T.ph = phi[L(N, x, y, z)];
+        phx = phi[L(N, xp, y, z)];
+        phy = phi[L(N, x, yp, z)];
+        phz = phi[L(N, x, y, zp)];
+        phxy = phi[L(N, xp, yp, z)];
+        phxz = phi[L(N, xp, y, zp)];
+        phyz = phi[L(N, x, yp, zp)];
+    }
+    T.plq01 = T.ph + phy + phx + phxy;
+    T.plq02 = T.ph + phz + phx + phxz;
+    T.plq12 = T.ph + phz + phy + phyz;
+    T.lnk0 = (F)1 + T.ph + phx;
+    T.lnk1 = (F)1 + T.ph + phy;
+    T.lnk2 = (F)1 + T.ph + phz;
+    return T;
+}
+
+// The dense cells of a chunk (T00 + Tij): lane (q, d) of a wavefront takes
+// cell q of the wavefront's group of 8 and the target at offset d from it,
+// and sums that target's partial (add_particle over the cell's records in
+// order, exactly the sum the target thread would form), then stores it in
+// the cell's record space (the layout above), and marks the
+// cell's count.  Each cell's 8 lanes sit in one wavefront, so its records
+// and count are read before they are overwritten.  The caller synchronises
+// the workgroup before and after.
+template <int KIND, bool USE_PHI, typename F>
+__device__ __forceinline__ void dense_partials(Stage<KIND, F> &st, const uint16_t *s_dlist, int ndense,
+                                               const uint32_t *s_off, uint32_t *s_cnt, uint32_t base,
+                                               const F *__restrict__ phi, int N, int x0, int y0, int z0, int nsx,
+                                               int nsy, int nsz, int nx, int ny, int nz) {
+    if constexpr (KIND == 3) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int q = lane >> 3, d = lane & 7;
+        const int dxx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+        for (int g = wave * 8; g < ndense; g += TPB / 8) {
+            const int k = g + q;
+            const int c = k < ndense ? (int)s_dlist[k] : 0;
+            const int px = c % nsx, py = (c / nsx) % nsy, pz = c / (nsx * nsy);
+            // source slot s along an axis is coordinate lo0 - 1 + s; the
+            // target at offset 1 (0) along that axis is the next (same) one
+            const int lx = order_slot(px, x0, nsx) - 1 + dxx, ly = order_slot(py, y0, nsy) - 1 + dy;
+            const int lz = order_slot(pz, z0, nsz) - 1 + dz;
+            const bool on = k < ndense && lx >= 0 && lx < nx && ly >= 0 && ly < ny && lz >= 0 && lz < nz;
+            const uint32_t m = on ? s_cnt[c] : 0u, r0 = on ? s_off[c] - base : 0u;
+            Target<F> T;
+            T.ph = T.plq01 = T.plq02 = T.plq12 = (F)0;
+            T.lnk0 = T.lnk1 = T.lnk2 = (F)1;
+            if (on) T = load_target<USE_PHI>(phi, N, x0 + lx, y0 + ly, z0 + lz);
+            Partial<F> P;
+            P.clear();
+#pragma unroll 1
+            for (uint32_t i = 0; i < m; i++) add_particle<KIND>(P, st.get((int)(r0 + i)), T, dxx, dy, dz);
+            // every lane of this wavefront has read its cell's records
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (on) {
+                const int a0 = 4 * (d >> 2), row = (int)r0 + (d & 3);
+                st.slot(a0, row) = P.lc;
+                st.slot(a0 + 1, row) = P.ld0;
+                st.slot(a0 + 2, row) = P.ld1;
+                st.slot(a0 + 3, row) = P.ld2;
+                if (dxx == 0 && dy == 0) st.slot(8, (int)r0 + dz) = P.lo01;
+                if (dxx == 0 && dz == 0) st.slot(9, (int)r0 + dy) = P.lo02;
+                if (dy == 0 && dz == 0) st.slot(10, (int)r0 + dxx) = P.lo12;
+            }
+            // the cell's count tells the target threads to read the slots
+            if (k < ndense && d == 0) s_cnt[c] |= DENSE_MARK;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
 // the particles 0 .. m-1 of the records, in order, onto one partial sum
 template <int KIND, int DX, int DY, int DZ, typename ST, typename R>
 __device__ __forceinline__ void add_records(Partial<R> &P, const ST &st, const Target<R> &T, uint32_t m) {
@@ -390,8 +532,11 @@ __device__ __forceinline__ void add_records(Partial<R> &P, const ST &st, const T
 // are compile-time constants (index arithmetic without integer division).
 // BATCH: the launch covers the boxes of a batch (its own instantiation, so
 // the one-box kernel is unchanged).
+// (FP64, N a multiple of the tile: the register allocator is held to 3
+// wavefronts per SIMD, which the kernel's LDS allows anyway; left to itself
+// it gives the dense phase 2.)
 template <int KIND, bool USE_PHI, bool ACCUM, bool POW2, bool FULL, typename F, bool BATCH = false>
-__global__ __launch_bounds__(TPB) void k_deposit_tiled(
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(F) == 8 && FULL ? 3 : 1, 8))) void k_deposit_tiled(
     const double *__restrict__ pos, const double *__restrict__ q, int64_t cap,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
     const F *__restrict__ phi, F *__restrict__ out_s, F *__restrict__ out_t, int N,
@@ -434,6 +579,14 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     __shared__ uint32_t s_cnt[NSRC];
     __shared__ uint32_t s_scan[TPB / 64];
     __shared__ uint16_t s_rc[CAP];
+    // the chunk's dense source cells (T00 + Tij), in no particular order (not
+    // in the batched kernel, whose prologue leaves no registers for the
+    // dense phase at 3 wavefronts per SIMD)
+    constexpr int DENSE = dense_min<F>();
+    constexpr bool DEN = KIND == 3 && DENSE > 0 && !BATCH;
+    __shared__ uint16_t s_dlist[DEN ? CAP / (DEN ? DENSE : 1) + 1 : 1];
+    __shared__ uint32_t s_nd, s_dmax;
+    if (DEN && threadIdx.x == 0) s_nd = s_dmax = 0;
 
     // XCD-aware tile order: consecutive workgroups of one XCD (dispatched
     // round-robin over the 8 XCDs) take neighbouring tiles and share L2
@@ -464,24 +617,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
     const F mass_sf = (F)mass_s;
     const uint32_t t = active ? L(N, x, y, z) : 0u;
     if (active) {
-        F phx = 0, phy = 0, phz = 0, phxy = 0, phxz = 0, phyz = 0;
-        if (USE_PHI) {
-            const int xp = x == N - 1 ? 0 : x + 1, yp = y == N - 1 ? 0 : y + 1, zp = z == N - 1 ? 0 : z + 1;
-            T.ph = phi[t];
-            phx = phi[L(N, xp, y, z)];
-            phy = phi[L(N, x, yp, z)];
-            phz = phi[L(N, x, y, zp)];
-            phxy = phi[L(N, xp, yp, z)];
-            phxz = phi[L(N, xp, y, zp)];
-            phyz = phi[L(N, x, yp, zp)];
-        }
-        // plaquette sums (gevolution.hpp:1263-1272) and link factors (1129-1144)
-        T.plq01 = T.ph + phy + phx + phxy;
-        T.plq02 = T.ph + phz + phx + phxz;
-        T.plq12 = T.ph + phz + phy + phyz;
-        T.lnk0 = (F)1 + T.ph + phx;
-        T.lnk1 = (F)1 + T.ph + phy;
-        T.lnk2 = (F)1 + T.ph + phz;
+        T = load_target<USE_PHI>(phi, N, x, y, z);
         if (ACCUM) {
             if (K::T00) A.s00 = out_s[t];
             if (K::TIJ) {
@@ -555,6 +691,11 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                 for (int k = 0; k < SX; k++) {
                     s_off[i * SX + k] = sv[k] - d;
                     s_cnt[i * SX + k] = sv[k + 1] - sv[k];
+                    // (s_nd was cleared above by lane 0 of this wavefront)
+                    if (DEN && sv[k + 1] - sv[k] >= (uint32_t)DENSE) {
+                        s_dlist[atomicAdd(&s_nd, 1u)] = (uint16_t)(i * SX + k);
+                        atomicMax(&s_dmax, sv[k + 1] - sv[k]);
+                    }
                 }
             }
             PQ pq[PR];
@@ -572,6 +713,11 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                 }
             }
             __syncthreads();
+            if (DEN && (uint32_t)__builtin_amdgcn_readfirstlane(s_dmax) >= (uint32_t)GEV_DEP_DENSE_TRIG) {
+                dense_partials<KIND, USE_PHI>(st, s_dlist, (int)s_nd, s_off, s_cnt, 0u, phi, N, x0, y0, z0, SX, SY,
+                                              SZ, TX, TY, TZ);
+                __syncthreads();
+            }
             if (active) {
                 const int cc = (lz * SY + ly) * SX + lx;  // source (dx, dy, dz) = (1, 1, 1)
                 add_source_cell<KIND, 1, 1, 1, false>(A, st, T, cc, 0, NSRC, 0u, s_off, s_cnt, mass_sf);
@@ -830,6 +976,10 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             for (int c = cb + threadIdx.x; c < ce; c += TPB) {
                 const uint32_t r0 = s_off[c] - base, n = s_cnt[c];
                 for (uint32_t i = 0; i < n; i++) s_rc[r0 + i] = (uint16_t)c;
+                if (DEN && n >= (uint32_t)DENSE) {
+                    s_dlist[atomicAdd(&s_nd, 1u)] = (uint16_t)c;
+                    atomicMax(&s_dmax, n);
+                }
             }
             __syncthreads();
             constexpr int PR = (CAP + TPB - 1) / TPB;
@@ -862,6 +1012,11 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
                 }
             }
             __syncthreads();
+            if (DEN && (uint32_t)__builtin_amdgcn_readfirstlane(s_dmax) >= (uint32_t)GEV_DEP_DENSE_TRIG) {
+                dense_partials<KIND, USE_PHI>(st, s_dlist, (int)s_nd, s_off, s_cnt, base, phi, N, x0, y0, z0, nsx,
+                                              nsy, nsz, nx, ny, nz);
+                __syncthreads();
+            }
             if (active && interior) {
                 // tiles away from coordinate 0: the 8 sources come in the
                 // fixed order (dz, dy, dx) = 111, 110 .. 000 reversed per
@@ -933,6 +1088,7 @@ __global__ __launch_bounds__(TPB) void k_deposit_tiled(
             }
 #undef GEV_ADD
         }
+        if (DEN && threadIdx.x == 0) s_nd = s_dmax = 0;  // every thread has read them
         __syncthreads();
         cb = ce;
     }

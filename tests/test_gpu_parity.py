@@ -248,6 +248,46 @@ def test_dense_cells_filing_and_deposit(gev, orc):
             b.moveParticles(gev.POS_GR, 0.002, 3, [a, a * a * N])
 
 
+@pytest.mark.parametrize("N", [64, 48])
+def test_clustered_tiles_deposit_bit_exact(gev, orc, N):
+    """Gaussian blobs a cell wide over a thin uniform background give cells
+    of 1 to several hundred particles: tiles whose source region fits the
+    LDS and tiles that take several chunks, interior tiles and tiles that
+    wrap (a blob sits on the origin), and many source cells at or above the
+    dense-cell size, whose partial sums are formed by (cell, target) lanes
+    before the target threads add them.  The fused T00 + Tij deposit and
+    Tij alone are bit-exact against the oracle."""
+    rng = np.random.default_rng(N + 1)
+    centres = np.concatenate([[[0.3 / N, 0.2 / N, 0.1 / N]], rng.uniform(0, 1, size=(40, 3))])
+    sizes = rng.integers(20, 1500, size=len(centres))
+    blobs = [c + rng.normal(0, 0.8 / N, size=(s, 3)) for c, s in zip(centres, sizes)]
+    pos = np.mod(np.concatenate(blobs + [rng.uniform(0, 1, size=(N ** 3 // 8, 3))]), 1.0)
+    pos[pos >= 1.0] = 0.0
+    n = len(pos)
+    ids = rng.permutation(n).astype(np.int64)
+    q = rng.standard_normal((n, 3)) * 1e-3
+    phi = smooth_field(N, rng, 1e-5)
+    a = 0.7
+    with gev.Box(N, n) as b:
+        b.upload_particles(pos, q, ids, 1e-3)
+        b.upload_field(gev.PHI, phi)
+        counts, _ = b.download_filing()
+        dpos, dq, dids = b.download_particles()
+        args = (N, dpos, dq, dids, 1e-3)
+        t00 = orc.projection_T00_project(*args, a=a, phi=phi)
+        tij = orc.projection_Tij_project(*args, a=a, phi=phi)
+        b.projection_init(gev.SOURCE)
+        b.projection_init(gev.SIJ)
+        b.projection_T00_Tij_project(a, 1.0)
+        assert np.array_equal(b.download_field(gev.SOURCE), t00)
+        assert np.array_equal(b.download_field(gev.SIJ), tij)
+        b.projection_init(gev.SIJ)
+        b.projection_Tij_project(a, True, 1.0)
+        assert np.array_equal(b.download_field(gev.SIJ), tij)
+        print("N=%d: %d particles, largest cell %d, %d cells of 5 or more" %
+              (N, n, counts.max(), int(np.count_nonzero(counts >= 5))), flush=True)
+
+
 def test_empty_box(gev):
     N = 16
     with gev.Box(N, 0) as b:
