@@ -185,6 +185,18 @@ def sim_setup(gev, N, boxsize):
     return sim, st
 
 
+def hbm_probe(gev):
+    """The device's achievable HBM bandwidth: the library's hand-written
+    streaming kernels over 2 GiB buffers (copy, copy with non-temporal hints,
+    read, write), GB/s and the fraction of the 8 TB/s peak."""
+    out = {}
+    for name, kind in (("copy", gev.PROBE_COPY), ("copy_nt", gev.PROBE_COPY_NT), ("read", gev.PROBE_READ),
+                       ("write", gev.PROBE_WRITE)):
+        gbs, ms = gev.bandwidth_probe(2 << 30, kind, 20)
+        out[name] = {"GBps": round(gbs, 1), "ms": round(ms, 4), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def host_info(threads):
     """The host the CPU baseline ran on: cores seen, CPU model, threads used
     and a STREAM-style triad bandwidth with those threads."""
@@ -513,7 +525,8 @@ def ensemble_report(args):
     over all of them, both stages of the cycle replayed from HIP graphs), the
     five batches on their own streams.  Against the same boxes run one after
     another alone (gev_run_to_outputs, same form); a sample of boxes is
-    checked bit for bit between the two."""
+    checked bit for bit between the two, and the first box of each grid size
+    is run in the FP64 form too: its z = 0 P(k) error."""
     import gevpm as gev
     import ensemble as en
     tpl = gev.load_homogeneous_template(os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat"))
@@ -532,9 +545,11 @@ def ensemble_report(args):
     steps = sum(r["particle_steps"] for r in res)
     nbytes = sum(stage_bytes(s.ngrid, s.ngrid ** 3, 4) * r["cycles"] for s, r in zip(specs, res))
     print("ensemble: batched run %.2f s, %.3e particle-steps/s" % (t_run, steps / t_run), file=sys.stderr, flush=True)
-    # the same boxes one after another, each alone
+    # the same boxes one after another, each alone; the first box of each
+    # grid size also in the FP64 form, for the precision form's P(k) error
     t_alone, same = 0.0, True
     check = set(range(0, len(specs), max(1, len(specs) // 8)))
+    prec = {}
     for s in specs:
         sim, st, ic, tabs = en.setup(s)
         with gev.Box(s.ngrid, s.ngrid ** 3) as b64, gev.Box(s.ngrid, s.ngrid ** 3, form=gev.FORM_FP32) as b:
@@ -544,6 +559,15 @@ def ensemble_report(args):
             t0 = time.perf_counter()
             out = b.run_to_outputs(sim, st, [0.0], s.ngrid // 2, s.boxsize)
             t_alone += time.perf_counter() - t0
+            if str(s.ngrid) not in prec:
+                sim64, st64, _, _ = en.setup(s)
+                o64 = b64.run_to_outputs(sim64, st64, [0.0], s.ngrid // 2, s.boxsize)
+                r = res[s.index]
+                low = (o64["occupation"][0] > 0) & (o64["k"][0] < 0.5 * np.pi * s.ngrid / s.boxsize)
+                prec[str(s.ngrid)] = {
+                    "box": s.index, "cycles_fp64": st64.cycle, "cycles_fp32": r["cycles"],
+                    "pk_max_rel_err_below_half_nyquist": float(np.max(np.abs(r["pk"][low] - o64["pk"][0][low]) /
+                                                                      np.abs(o64["pk"][0][low])))}
         if s.index in check:
             r = res[s.index]
             same = same and st.cycle == r["cycles"] and np.array_equal(out["pk"][0], r["pk"])
@@ -562,6 +586,7 @@ def ensemble_report(args):
                          "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(nbytes)},
             "boxes_alone_s": round(t_alone, 3), "speedup_vs_alone": round(t_alone / t_run, 3),
             "identical_to_alone_sample": bool(same), "per_grid": per_n,
+            "precision_vs_fp64": prec,
             "autotuned_fft_tiles": {str(N): t for N, t in tuned.items()},
             "cycles": sorted({r["cycles"] for r in res})}
 
@@ -597,13 +622,12 @@ def main():
     args = ap.parse_args()
 
     if args.calibrate_copy:
-        import torch
-        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+        # five launches of the library's hand-written 1 GiB copy (the PMC byte
+        # counters' scale), then the probes' rates
+        import gevpm as gev
+        gev.bandwidth_probe(1 << 30, gev.PROBE_COPY, 5)
+        print(json.dumps({"calibration": "copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                          "hbm_probe": hbm_probe(gev)}))
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -613,6 +637,9 @@ def main():
 
     import gevpm as gev
     gev.lib()
+    probe = hbm_probe(gev)
+    print("HBM probe: copy %.0f GB/s, read %.0f GB/s, write %.0f GB/s" %
+          (probe["copy"]["GBps"], probe["read"]["GBps"], probe["write"]["GBps"]), file=sys.stderr, flush=True)
 
     N = args.ngrid
     Np = N ** 3
@@ -734,6 +761,7 @@ def main():
                           "frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "min_bytes_per_step": cycle_bytes(N, Np)},
         "roofline": roofline,
+        "hbm_probe": probe,
         "cpu_baseline": cpu,
         "kernels": breakdown,
         "ic_generator": ic_info,

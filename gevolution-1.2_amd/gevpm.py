@@ -51,7 +51,7 @@ EXPORTED = [
     "gev_batch_footprint_bytes", "gev_batch_create", "gev_batch_destroy", "gev_batch_last_error", "gev_batch_box",
     "gev_batch_start_box", "gev_batch_stop_box", "gev_batch_get_state", "gev_batch_run_cycles",
     "gev_batch_set_outputs", "gev_batch_step_outputs", "gev_batch_run_to_outputs", "gev_batch_get_outputs",
-    "gev_batch_synchronize", "gev_batch_autotune",
+    "gev_batch_synchronize", "gev_batch_autotune", "gev_bandwidth_probe",
 ]
 
 
@@ -155,6 +155,7 @@ def lib():
         L.gev_timing_enable.argtypes = [_p, _i]
         L.gev_timing_reset.argtypes = [_p]
         L.gev_timing_report.argtypes = [_p, ctypes.c_char_p, _sz]
+        L.gev_bandwidth_probe.argtypes = [_i64, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_d)]
         L.gev_prng_fill.argtypes = [_p, ctypes.c_uint32, ctypes.c_uint64, _i64, _p]
         L.gev_load_homogeneous_template.argtypes = [ctypes.c_char_p, _p, _i64, ctypes.POINTER(_i64)]
         L.gev_load_transfer_functions.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _d, _d, _p, _p, _p,
@@ -536,6 +537,19 @@ class Box:
             name, calls, ms = line.split()
             out[name] = (int(calls), float(ms))
         return out
+
+
+PROBE_COPY, PROBE_COPY_NT, PROBE_READ, PROBE_WRITE = 0, 1, 2, 3
+
+
+def bandwidth_probe(nbytes=1 << 30, kind=PROBE_COPY, reps=20):
+    """(GB/s, ms per launch) of a hand-written streaming kernel over buffers
+    of nbytes: the device's achievable HBM bandwidth."""
+    g, m = _d(), _d()
+    rc = lib().gev_bandwidth_probe(int(nbytes), int(kind), int(reps), ctypes.byref(g), ctypes.byref(m))
+    if rc:
+        raise GevError("bandwidth_probe failed (%d)" % rc)
+    return g.value, m.value
 
 
 def batch_footprint_bytes(ngrid, npart, nbox, form=FORM_FP64, gr=1):

@@ -504,6 +504,17 @@ int gev_timing_reset(gev_ctx *ctx);
    len). Synchronous. */
 int gev_timing_report(gev_ctx *ctx, char *buf, size_t len);
 
+/* ---- bandwidth probe ----------------------------------------------------- */
+
+/* No counterpart in the reference: the device's achievable HBM bandwidth,
+   measured with hand-written streaming kernels over two buffers of `bytes`
+   each (allocated and freed here, on a stream of its own), `reps` timed
+   launches after one untimed.  *gbs = bytes moved per launch (read + written
+   for the copies) / mean launch time; *ms = mean launch time.  It is the
+   practical ceiling the kernels' HBM rooflines are read against. */
+enum { GEV_PROBE_COPY = 0, GEV_PROBE_COPY_NT = 1, GEV_PROBE_READ = 2, GEV_PROBE_WRITE = 3 };
+int gev_bandwidth_probe(int64_t bytes, int kind, int reps, double *gbs, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,3 +59,12 @@ def test_footprint_of_both_forms():
     assert f64 - f32 >= fields64 // 2 - N * 8  # every field element halves
     assert f32 >= fields64 // 2 + particles
     assert gevpm.footprint_bytes(N, cap, 5) < 0  # unknown form
+
+
+def test_bandwidth_probe_rejects_bad_arguments_before_any_device_work():
+    import ctypes
+    import gevpm
+    L = gevpm.lib()
+    g, m = ctypes.c_double(), ctypes.c_double()
+    for args in ((0, 0, 1), (1 << 30, 7, 1), (1 << 30, 0, 0), (1 << 30, -1, 5)):
+        assert L.gev_bandwidth_probe(*args, ctypes.byref(g), ctypes.byref(m)) == -1
