@@ -207,6 +207,7 @@ static void slab_layout(gev_ctx *c, const Sizes &s, SlabLayout &L) {
     L.add((void **)&c->red_u, 64);
     c->pk_bins = spectrum_bins_cap(c->N);
     L.add((void **)&c->pk_scratch, spectrum_scratch_doubles(c->N, c->pk_bins) * 8);
+    L.add((void **)&c->ic_stage, ic_stage_bytes(c->N));
 }
 
 // points c's device arrays at the slab starting at base (a box of a batch)

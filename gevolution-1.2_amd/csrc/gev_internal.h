@@ -18,6 +18,29 @@ constexpr int REC_SLOTS = 1024;
 
 namespace gev {
 
+// A store that streams past the caches (non-temporal) when nt is set: for
+// the large outputs no kernel reads back before they have left the caches
+// anyway; small boxes keep plain stores, whose data the next kernel may
+// still find in the caches.
+template <typename T> __device__ __forceinline__ void store_stream(T *p, T v, bool nt) {
+    if (nt) {
+        if constexpr (sizeof(T) == 16) {
+            using V = double __attribute__((ext_vector_type(2)));
+            __builtin_nontemporal_store(*reinterpret_cast<const V *>(&v), reinterpret_cast<V *>(p));
+        } else if constexpr (sizeof(T) == 8 && !__is_same(T, double) && !__is_same(T, int64_t)) {
+            using V = float __attribute__((ext_vector_type(2)));
+            __builtin_nontemporal_store(*reinterpret_cast<const V *>(&v), reinterpret_cast<V *>(p));
+        } else {
+            __builtin_nontemporal_store(v, p);
+        }
+    } else {
+        *p = v;
+    }
+}
+// the grid size from which the large outputs are streamed
+constexpr int STREAM_MIN_N = 256;
+
+
 // One complex k-space row holds kx in [0, N/2]; rows are padded to a multiple
 // of 8 complexes so that every row starts on a 128-byte boundary (FP64).
 inline int kpitch(int N) { return ((N / 2 + 1) + 7) / 8 * 8; }
@@ -170,6 +193,9 @@ struct gev_ctx {
     // up to pk_bins bins
     double *pk_scratch = nullptr;
     int pk_bins = 0;
+    // the IC generator's host-built tables, copied in stream order
+    // (ic_stage_bytes(N) per box)
+    char *ic_stage = nullptr;
 
     // reductions
     double *red_d = nullptr;              // device double scratch (sums)
@@ -278,6 +304,15 @@ int launch_ic_q(gev_ctx *c, const double *field, double coeff);         // max |
 // context's own scratch holds
 int64_t spectrum_scratch_doubles(int N, int numbins);
 inline int spectrum_bins_cap(int N) { return N / 2 > 64 ? N / 2 : 64; }
+// the IC staging area of a box of grid N: up to IC_STAGE_KNOTS spline knots
+// (x, y and the coefficients), the N-entry sinc row, N^2 row marks and a
+// crystal template of up to IC_STAGE_TEMPLATE particles, each 256-byte
+// aligned; larger tables are allocated per call
+constexpr int64_t IC_STAGE_KNOTS = 4096, IC_STAGE_TEMPLATE = 4096;
+inline int64_t ic_stage_bytes(int N) {
+    return 3 * (IC_STAGE_KNOTS * 8 + 256) + ((int64_t)N * 4 + 256) + ((int64_t)N * N + 256) +
+           (IC_STAGE_TEMPLATE * 12 + 256);
+}
 int spectrum(gev_ctx *c, const double2 *f1, const double2 *f2, int ncomp, int mode, int comp1, int comp2,
              int numbins, int deconvolve, int ktype, double *kbin, double *power, double *kscatter,
              double *pscatter, int32_t *occupation);

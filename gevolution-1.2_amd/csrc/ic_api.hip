@@ -15,17 +15,39 @@ using namespace gev;
 
 namespace {
 
-// device allocation released at scope exit (IC set-up only; not the cycle path)
+// A device table of the IC set-up: a slice of the box's IC staging area, or
+// (a table larger than the area holds) an allocation released at scope exit
 struct DevMem {
     void *p = nullptr;
+    bool owned = false;
     ~DevMem() {
-        if (p) (void)hipFree(p);
+        if (owned && p) (void)hipFree(p);
     }
 };
 
-int dev_copy(gev_ctx *c, DevMem &m, const void *host, size_t bytes, const char *what) {
-    int rc = check_hip(c, hipMalloc(&m.p, bytes > 0 ? bytes : 1), what);
-    if (!rc && bytes > 0) rc = check_hip(c, hipMemcpy(m.p, host, bytes, hipMemcpyHostToDevice), what);
+// consecutive slices of the staging area, 256-byte aligned
+struct Stage {
+    char *at;
+    int64_t left;
+    explicit Stage(gev_ctx *c) : at(c->ic_stage), left(ic_stage_bytes(c->N)) {}
+    int take(gev_ctx *c, DevMem &m, size_t bytes, const char *what) {
+        const int64_t need = ((int64_t)bytes + 255) / 256 * 256;
+        if (need <= left) {
+            m.p = at;
+            at += need;
+            left -= need;
+            return 0;
+        }
+        m.owned = true;
+        return check_hip(c, hipMalloc(&m.p, bytes > 0 ? bytes : 1), what);
+    }
+};
+
+// host table -> device, in the context's stream order (the callers
+// synchronise the stream before their host tables go out of scope)
+int dev_copy(gev_ctx *c, Stage &st, DevMem &m, const void *host, size_t bytes, const char *what) {
+    int rc = st.take(c, m, bytes, what);
+    if (!rc && bytes > 0) rc = check_hip(c, hipMemcpyAsync(m.p, host, bytes, hipMemcpyHostToDevice, c->stream), what);
     return rc;
 }
 
@@ -212,6 +234,7 @@ GEV_EXPORT int gev_prng_fill(gev_ctx *c, uint32_t seed, uint64_t pos0, int64_t n
     if (!c || n < 0 || (n > 0 && !host_out)) return c ? fail(c, GEV_EINVAL, "bad prng arguments") : GEV_EINVAL;
     if (n == 0) return 0;
     DevMem d;
+    d.owned = true;
     int rc = check_hip(c, hipMalloc(&d.p, (size_t)n * 4), "prng buffer");
     if (!rc) rc = launch_prng_fill(c, seed, pos0, n, (uint32_t *)d.p);
     if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "prng sync");
@@ -354,12 +377,16 @@ GEV_EXPORT int gev_generate_displacement_field(gev_ctx *c, double coeff, int32_t
     for (int i = 1; i < N; i++)
         sinc[i] = deconvolve_f == 1 ? (float)(sin(M_PI * (float)i / (float)N) * (float)N / (M_PI * (float)i)) : 1.f;
     DevMem dx, dy, dc, ds, dr;
-    int rc = dev_copy(c, dx, x, nknots * 8, "spline x");
-    if (!rc) rc = dev_copy(c, dy, y, nknots * 8, "spline y");
-    if (!rc) rc = dev_copy(c, dc, cc.data(), nknots * 8, "spline c");
-    if (!rc) rc = dev_copy(c, ds, sinc.data(), N * 4, "sinc");
-    if (!rc) rc = check_hip(c, hipMalloc(&dr.p, (size_t)N * N), "row marks");
-    if (rc) return rc;
+    Stage stage(c);
+    int rc = dev_copy(c, stage, dx, x, nknots * 8, "spline x");
+    if (!rc) rc = dev_copy(c, stage, dy, y, nknots * 8, "spline y");
+    if (!rc) rc = dev_copy(c, stage, dc, cc.data(), nknots * 8, "spline c");
+    if (!rc) rc = dev_copy(c, stage, ds, sinc.data(), N * 4, "sinc");
+    if (!rc) rc = stage.take(c, dr, (size_t)N * N, "row marks");
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);  // the host tables outlive the copies
+        return rc;
+    }
     RealizeSetup s{(uint8_t *)dr.p, (const float *)ds.p, (const double *)dx.p, (const double *)dy.p,
                    (const double *)dc.p, nknots, seed, ksphere, ignorekernel, coeff};
     rc = launch_realize(c, s);
@@ -377,8 +404,12 @@ GEV_EXPORT int gev_initialize_particle_positions(gev_ctx *c, int64_t numpart, co
         if (!(partdata[i] >= 0.f && partdata[i] < 1.f))
             return fail(c, GEV_EINVAL, "template positions must lie in [0,1)");
     DevMem t;
-    int rc = dev_copy(c, t, partdata, (size_t)numpart * 12, "template");
-    if (rc) return rc;
+    Stage stage(c);
+    int rc = dev_copy(c, stage, t, partdata, (size_t)numpart * 12, "template");
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);
+        return rc;
+    }
     c->n = n;
     c->pmass = mass;
     rc = launch_init_particles(c, (const float *)t.p, numpart, numtile);
