@@ -833,6 +833,12 @@ struct StepArgs {
 #ifndef GEV_KICK_NT
 #define GEV_KICK_NT 1
 #endif
+// whole tiles away from the box faces compute their staging indices as one
+// offset from the region's corner (no clamp, no wrap): 6.46 -> 5.90 ms per
+// FP64 kick-drift at 512^3 early, 4.69 -> 4.49 ms in the FP32 form
+#ifndef GEV_KICK_INNER
+#define GEV_KICK_INNER 1
+#endif
 #ifndef GEV_KICK_WAVES_F32
 #define GEV_KICK_WAVES_F32 5
 #endif
@@ -963,8 +969,18 @@ __global__ __launch_bounds__(TPB, (sizeof(F) == 4 && !BATCH ? GEV_KICK_WAVES_F32
     // region is clamped there: the coordinate stays within [x0 - 1, x0 + nx]
     // (x0 + nx <= N), which one periodic step maps into [0, N).  The slots
     // beyond hold copies of the last staged cell and are never read.
+    // A whole tile away from the box faces (no coordinate of its regions
+    // wraps, nothing to clamp) takes its cells as one offset from the
+    // region's corner.
+    const bool inner = nx == KTX && ny == KTY && nz == KTZ && x0 > 0 && y0 > 0 && z0 > 0 && x0 + KTX < N &&
+                       y0 + KTY < N && z0 + KTZ < N;
     auto cell_of = [&](auto reg, int i) {
         using R = decltype(reg);
+        if (GEV_KICK_INNER && inner) {
+            const uint32_t corner = (uint32_t)L(N, x0 - R::OX, y0 - R::OY, z0 - R::OZ);  // N^3 < 2^32
+            return (int64_t)(corner + (uint32_t)(i % R::SX) +
+                             (uint32_t)N * ((uint32_t)((i / R::SX) % R::SY) + (uint32_t)N * (uint32_t)(i / (R::SX * R::SY))));
+        }
         const int rx = min(i % R::SX, nx + R::OX), ry = min((i / R::SX) % R::SY, ny + R::OY);
         const int rz = min(i / (R::SX * R::SY), nz + R::OZ);
         return L(N, wrapi(x0 - R::OX + rx, N), wrapi(y0 - R::OY + ry, N), wrapi(z0 - R::OZ + rz, N));
