@@ -58,10 +58,23 @@ constexpr int NSRC = SX * SY * SZ;  // source cells of a tile (425 for 16x4x4)
 #define GEV_DEP_CAP_F32 448
 #endif
 template <typename R> constexpr int dep_cap() { return sizeof(R) == 4 ? GEV_DEP_CAP_F32 : GEV_DEP_CAP; }
-// the FP32 form's tiles above this many particles in their source region
-// take the fixed-point particle-parallel path
+// The FP32 form's tiles above this many particles in their source region
+// take the fixed-point particle-parallel path, and so do its tiles above CAP
+// particles whose region has a cell of at least GEV_DEP_FIXED_CELL (0: none);
+// the other tiles above CAP take the chunked gather.  In the fixed-point
+// path, a region with a cell of at least GEV_FX_SPREAD particles (0: never)
+// spreads each wavefront's lanes over the tile's records (the pass 2 loop).
+// At 512^3 these take the FP32 deposit from 8.09 to 7.64 ms at z = 3 and
+// from 13.02 to 11.58 ms at z = 0 (448 / 0 / 0: the fixed-point path for
+// every tile above CAP, records in order).
 #ifndef GEV_DEP_FIXED_MIN
-#define GEV_DEP_FIXED_MIN 448
+#define GEV_DEP_FIXED_MIN 896
+#endif
+#ifndef GEV_DEP_FIXED_CELL
+#define GEV_DEP_FIXED_CELL 16
+#endif
+#ifndef GEV_FX_SPREAD
+#define GEV_FX_SPREAD 16
 #endif
 // Source cells with at least this many particles in a chunk are "dense"
 // (T00 + Tij deposit): their partial sums for all their in-tile targets are
@@ -582,6 +595,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(F) =
     __shared__ uint32_t s_gbeg[NSRC];
     __shared__ uint32_t s_cnt[NSRC];
     __shared__ uint32_t s_scan[TPB / 64];
+    __shared__ uint32_t s_rmax;  // FP32 form: the region's largest source cell
+    if (sizeof(F) == 4 && threadIdx.x == 0) s_rmax = 0;
     __shared__ uint16_t s_rc[CAP];
     // the chunk's dense source cells (T00 + Tij), in no particular order (not
     // in the batched kernel, whose prologue leaves no registers for the
@@ -783,6 +798,16 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(F) =
             if (lane >= off) incl += u;
         }
         if (lane == 63) s_scan[wave] = incl;
+        if constexpr (sizeof(F) == 4) {
+            // the largest source cell of the region (the fixed-point path's record order)
+            uint32_t mx = c0 > c1 ? c0 : c1;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint32_t o = (uint32_t)__shfl_xor((int)mx, off);
+                mx = o > mx ? o : mx;
+            }
+            if (lane == 0) atomicMax(&s_rmax, mx);
+        }
         __syncthreads();
         uint32_t before = 0;
 #pragma unroll
@@ -796,7 +821,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(F) =
     }
 
     // FP32 form, a tile whose source region holds more than CAP particles
-    // (the clustered tiles of late cycles): particle-parallel accumulation
+    // and is crowded or has a dense cell (the clustered tiles of late
+    // cycles; see GEV_DEP_FIXED_MIN): particle-parallel accumulation
     // into 64-bit fixed point in LDS.  In the gather each target thread walks
     // its 8 source cells' particles, so a dense cell keeps its wavefront busy
     // while the other lanes idle; here every thread takes particles in turn
@@ -806,7 +832,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(F) =
     // tile's fixed-point scale for its component, a power of two chosen so
     // that the sum of all the tile's contribution bounds stays below 2^62.
     if constexpr (sizeof(F) == 4 && (KIND == 1 || KIND == 2 || KIND == 3)) {
-        if (s_off[nsrc] > (uint32_t)max(CAP, GEV_DEP_FIXED_MIN)) {
+        if (s_off[nsrc] > (uint32_t)max(CAP, GEV_DEP_FIXED_MIN) ||
+            (GEV_DEP_FIXED_CELL > 0 && s_off[nsrc] > (uint32_t)CAP && s_rmax >= (uint32_t)GEV_DEP_FIXED_CELL)) {
             constexpr int NCOMP = 7;  // T00, T_00, T_01, T_02, T_11, T_12, T_22
             unsigned long long *s_acc = su.fx.acc;
             float *s_T = su.fx.T;  // ph, plq01, plq02, plq12 of every target
@@ -898,8 +925,18 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(F) =
             float sc[NCOMP];
 #pragma unroll
             for (int cc = 0; cc < NCOMP; cc++) sc[cc] = ldexpf(1.f, s_k[cc]);
-            // pass 2: every record's contributions to its in-tile targets
-            for (uint32_t r = threadIdx.x; r < total; r += TPB) {
+            // pass 2: every record's contributions to its in-tile targets.
+            // In a tile with a dense cell, lane l of a wavefront takes record
+            // l * K + it at its step it (K = ceil(total / 64)), so the lanes
+            // of one atomic add sit K records apart rather than in the dense
+            // cell, whose particles all add to the same 8 targets
+            const bool spread = GEV_FX_SPREAD > 0 && s_rmax >= (uint32_t)GEV_FX_SPREAD;
+            const uint32_t KSP = (total + 63) / 64;
+            const uint32_t r_first = spread ? (threadIdx.x & 63) * KSP + (threadIdx.x >> 6) : threadIdx.x;
+            const uint32_t r_step = spread ? TPB / 64 : TPB;
+            const uint32_t r_end = spread ? (threadIdx.x & 63) * KSP + KSP : total;
+            for (uint32_t r = r_first; r < r_end; r += r_step) {
+                if (r >= total) break;
                 int c, sx, sy, sz;
                 uint32_t pidx;
                 locate(r, c, sx, sy, sz, pidx);

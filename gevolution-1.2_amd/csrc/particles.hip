@@ -225,7 +225,20 @@ __global__ __launch_bounds__(TPB) void k_scatter(const uint32_t *__restrict__ ce
 // workgroup per cell).  (A thread-per-cell insertion sort over global memory
 // made dense cells slow: every comparison was two dependent global loads.)
 constexpr uint32_t SORT_TINY = 4;
-constexpr int SORT_SEG = 2048;  // permutation entries of a segment sorted in LDS
+// Permutation entries of a segment sorted in LDS.  1024 (17 KB of LDS) lets
+// 8 workgroups share a CU, twice as many as 2048: the segment sort is
+// latency-bound (cell starts, then permutation and IDs, then the ranks), and
+// at 512^3, z = 0 it takes 1.64 against 2.85 ms; the few denser segments
+// go to k_sort_big.
+#ifndef GEV_SORT_SEG
+#define GEV_SORT_SEG 1024
+#endif
+constexpr int SORT_SEG = GEV_SORT_SEG;
+// the next segment's cell starts are loaded while a segment is ranked
+// (1.56 against 1.64 ms at 512^3, z = 0)
+#ifndef GEV_SORT_PF
+#define GEV_SORT_PF 1
+#endif
 static_assert(TPB <= 256, "a segment's cell index fits 8 bits");
 constexpr int SORT_LDS = 2048;  // particles of a listed cell k_sort_big ranks in LDS
 
@@ -333,14 +346,37 @@ __global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restric
         if (threadIdx.x == 0) s_mask = m;
     }
     __syncthreads();
+    // GEV_SORT_PF: the cell starts of the next flagged segment are loaded
+    // while this one is ranked (thread t: cell t, thread 0 also the end;
+    // a segment has at most TPB cells)
+    uint32_t pf_s = 0, pf_e = 0;
+    auto prefetch = [&](unsigned long long m) {
+        if (m == 0ull) return;
+        const int64_t c = (int64_t)(blockIdx.x + (uint32_t)(__ffsll((long long)m) - 1) * gridDim.x) * TPB;
+        const int nc = (int)(n3v - c < TPB ? n3v - c : TPB);
+        if ((int)threadIdx.x <= nc) pf_s = start[c + threadIdx.x];
+        if (threadIdx.x == 0) pf_e = start[c + nc];
+    };
+    if (GEV_SORT_PF) prefetch(s_mask);
     for (unsigned long long mask = s_mask; mask != 0ull; mask &= mask - 1ull) {
         const uint32_t e = blockIdx.x + (uint32_t)(__ffsll((long long)mask) - 1) * gridDim.x;
         const int64_t c0 = (int64_t)e * TPB;
         const int ncell = (int)(n3v - c0 < TPB ? n3v - c0 : TPB);
-        const uint32_t s0 = start[c0];
-        for (int t = threadIdx.x; t <= ncell; t += TPB) s_cs[t] = start[c0 + t] - s0;
+        uint32_t s0 = 0;
+        if (GEV_SORT_PF) {
+            // absolute starts (start[c0] is thread 0's value, read back
+            // after the barrier)
+            if ((int)threadIdx.x <= ncell) s_cs[threadIdx.x] = pf_s;
+            if (threadIdx.x == 0) s_cs[ncell] = pf_e;
+            prefetch(mask & (mask - 1ull));
+        } else {
+            s0 = start[c0];
+            for (int t = threadIdx.x; t <= ncell; t += TPB) s_cs[t] = start[c0 + t] - s0;
+        }
         __syncthreads();
-        const uint32_t total = s_cs[ncell];
+        if (GEV_SORT_PF) s0 = s_cs[0];
+        const uint32_t sb = GEV_SORT_PF ? s0 : 0u;  // s_cs[t] - sb: cell t's start in the segment
+        const uint32_t total = s_cs[ncell] - sb;
         if (total > (uint32_t)SORT_SEG) {
             // dense segment: its larger cells are ranked one workgroup each
             if (threadIdx.x < ncell && s_cs[threadIdx.x + 1] - s_cs[threadIdx.x] > SORT_TINY)
@@ -352,7 +388,7 @@ __global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restric
             int lo = 0, hi = ncell - 1;  // the last cell whose start is <= i
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (s_cs[mid] <= i) lo = mid;
+                if (s_cs[mid] - sb <= i) lo = mid;
                 else hi = mid - 1;
             }
             return lo;
@@ -369,7 +405,7 @@ __global__ __launch_bounds__(TPB) void k_sort_segments(const uint32_t *__restric
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < total; i += TPB) {
             const int c = s_cell[i];
-            const uint32_t b = s_cs[c], f = s_cs[c + 1];
+            const uint32_t b = s_cs[c] - sb, f = s_cs[c + 1] - sb;
             if (f - b <= SORT_TINY) continue;
             const int64_t k = s_key[i];
             uint32_t r = 0;
@@ -836,6 +872,9 @@ struct StepArgs {
 // whole tiles away from the box faces compute their staging indices as one
 // offset from the region's corner (no clamp, no wrap): 6.46 -> 5.90 ms per
 // FP64 kick-drift at 512^3 early, 4.69 -> 4.49 ms in the FP32 form
+#ifndef GEV_KICK_RUN
+#define GEV_KICK_RUN 0
+#endif
 #ifndef GEV_KICK_INNER
 #define GEV_KICK_INNER 1
 #endif
@@ -1106,7 +1145,13 @@ __global__ __launch_bounds__(TPB, (sizeof(F) == 4 && !BATCH ? GEV_KICK_WAVES_F32
             const uint32_t c = (uint32_t)L(N, cell1<POW2>(x[0], dx, N), cell1<POW2>(x[1], dx, N),
                                            cell1<POW2>(x[2], dx, N));
             cell[w] = c;
-            atomicAdd(count + c, 1u);  // no return value: the wave does not wait for it
+            if (GEV_KICK_RUN) {
+                // one count per run of equal new cells in the wavefront
+                const CellRun run = cell_run(c);
+                if (run.off == 0) atomicAdd(count + c, run.len);
+            } else {
+                atomicAdd(count + c, 1u);  // no return value: the wave does not wait for it
+            }
         }
         // the next particle was fetched one step ahead; fetch the one after
         j += TPB;

@@ -288,6 +288,42 @@ def test_fp32_dense_tiles(gev):
         assert rel(f32a[1][c], f64[1][c]) <= 1e-5, (c, rel(f32a[1][c], f64[1][c]))
 
 
+def test_fp32_tiles_above_the_records(gev):
+    """FP32 tiles whose source region holds more particles than the LDS
+    records take one of two paths: the chunked gather when every source cell
+    is small (2 particles a cell: 850 in a region), the fixed-point path when
+    a cell is dense (a cell of 40 on top of that) or the region is crowded
+    (4 a cell in one corner of the box; its records spread over the lanes).
+    Each within float rounding of the FP64 deposit, bit-reproducible."""
+    N = 32
+    rng = np.random.default_rng(11)
+    g = (np.stack(np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij"), -1).reshape(-1, 3) + 0.5) / N
+    pos = [np.repeat(g, 2, axis=0) + rng.uniform(-0.45, 0.45, size=(2 * N ** 3, 3)) / N]
+    corner = g[np.all(g < 0.25, axis=1)]
+    pos.append(np.repeat(corner, 2, axis=0) + rng.uniform(-0.45, 0.45, size=(2 * len(corner), 3)) / N)
+    pos.append(np.array([[0.6, 0.6, 0.6]]) + rng.uniform(0.001, 0.03, size=(40, 3)) / N)
+    pos = np.mod(np.concatenate(pos), 1.0)
+    n = len(pos)
+    ids = rng.permutation(np.arange(n, dtype=np.int64))
+    q = rng.standard_normal((n, 3)) * 1e-3
+    phi = rng.standard_normal((N, N, N)) * 1e-5
+    out = []
+    for form in (gev.FORM_FP64, gev.FORM_FP32, gev.FORM_FP32):
+        with gev.Box(N, n, form=form) as b:
+            b.upload_particles(pos, q, ids, 1e-3)
+            b.upload_field(gev.PHI, phi)
+            b.projection_init(gev.SOURCE)
+            b.projection_init(gev.SIJ)
+            b.projection_T00_Tij_project(0.4, 1.0)
+            out.append((b.download_field(gev.SOURCE), b.download_field(gev.SIJ)))
+    f64, f32a, f32b = out
+    for x, y in zip(f32a, f32b):
+        assert np.array_equal(x, y)  # run to run
+    assert rel(f32a[0], f64[0]) <= 1e-6, rel(f32a[0], f64[0])
+    for c in range(6):
+        assert rel(f32a[1][c], f64[1][c]) <= 1e-5, (c, rel(f32a[1][c], f64[1][c]))
+
+
 def test_fp32_form_arguments(gev):
     with pytest.raises(gev.GevError):
         gev.Box(32, 8, form=7)
