@@ -872,9 +872,6 @@ struct StepArgs {
 // whole tiles away from the box faces compute their staging indices as one
 // offset from the region's corner (no clamp, no wrap): 6.46 -> 5.90 ms per
 // FP64 kick-drift at 512^3 early, 4.69 -> 4.49 ms in the FP32 form
-#ifndef GEV_KICK_RUN
-#define GEV_KICK_RUN 0
-#endif
 #ifndef GEV_KICK_INNER
 #define GEV_KICK_INNER 1
 #endif
@@ -1145,13 +1142,10 @@ __global__ __launch_bounds__(TPB, (sizeof(F) == 4 && !BATCH ? GEV_KICK_WAVES_F32
             const uint32_t c = (uint32_t)L(N, cell1<POW2>(x[0], dx, N), cell1<POW2>(x[1], dx, N),
                                            cell1<POW2>(x[2], dx, N));
             cell[w] = c;
-            if (GEV_KICK_RUN) {
-                // one count per run of equal new cells in the wavefront
-                const CellRun run = cell_run(c);
-                if (run.off == 0) atomicAdd(count + c, run.len);
-            } else {
-                atomicAdd(count + c, 1u);  // no return value: the wave does not wait for it
-            }
+            // no return value: the wave does not wait for it (one atomic per
+            // run of equal cells, as in the scatter, was slower: 6.49 against
+            // 6.02 ms early at 512^3, faster only near z = 0)
+            atomicAdd(count + c, 1u);
         }
         // the next particle was fetched one step ahead; fetch the one after
         j += TPB;
