@@ -799,7 +799,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(F) =
         }
         if (lane == 63) s_scan[wave] = incl;
         if constexpr (sizeof(F) == 4) {
-            // the largest source cell of the region (the fixed-point path's record order)
+            // the largest source cell of the region (the FP32 form's path and record order)
             uint32_t mx = c0 > c1 ? c0 : c1;
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
