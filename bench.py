@@ -591,7 +591,7 @@ def ensemble_report(args):
             "cycles": sorted({r["cycles"] for r in res})}
 
 
-def main():
+def arg_parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=117, help="timed cycles from z = 100 (117: the whole run to z = 0)")
@@ -619,7 +619,11 @@ def main():
     ap.add_argument("--ensemble-boxes", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 1 GiB device copies (scale for the PMC byte counters)")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = arg_parser().parse_args()
 
     if args.calibrate_copy:
         # five launches of the library's hand-written 1 GiB copy (the PMC byte

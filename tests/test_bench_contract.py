@@ -47,3 +47,20 @@ def test_more_than_one_gpu_exits_with_a_message():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+def test_profile_commands_parse():
+    """Every bench.py command quoted in a committed profile summary is one the
+    bench still accepts (the summaries name the command that made them)."""
+    import re
+    pdir = os.path.join(ROOT, "profiles")
+    seen = 0
+    for name in sorted(os.listdir(pdir)):
+        for line in open(os.path.join(pdir, name)):
+            m = re.search(r"bench\.py((?:\s+--[\w-]+(?:\s+[^\s(-][^\s]*)?)*)", line)
+            if not m or "--" not in m.group(1):
+                continue
+            argv = m.group(1).split()
+            bench.arg_parser().parse_args(argv)  # exits (SystemExit) on an unknown flag
+            seen += 1
+    assert seen >= 4, seen
