@@ -288,14 +288,16 @@ def test_fp32_dense_tiles(gev):
         assert rel(f32a[1][c], f64[1][c]) <= 1e-5, (c, rel(f32a[1][c], f64[1][c]))
 
 
-def test_fp32_tiles_above_the_records(gev):
+@pytest.mark.parametrize("N", [24, 32])
+def test_fp32_tiles_above_the_records(gev, N):
     """FP32 tiles whose source region holds more particles than the LDS
     records take one of two paths: the chunked gather when every source cell
     is small (2 particles a cell: 850 in a region), the fixed-point path when
-    a cell is dense (a cell of 40 on top of that) or the region is crowded
-    (4 a cell in one corner of the box; its records spread over the lanes).
-    Each within float rounding of the FP64 deposit, bit-reproducible."""
-    N = 32
+    a cell is dense (a cell of 40 on top of that; its records spread over the
+    lanes) or the region is crowded (4 a cell in one corner of the box).
+    Each within float rounding of the FP64 deposit, bit-reproducible.  At
+    N = 24 the x tiles of 16 and 8 cells take the general (not full-tile)
+    paths."""
     rng = np.random.default_rng(11)
     g = (np.stack(np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij"), -1).reshape(-1, 3) + 0.5) / N
     pos = [np.repeat(g, 2, axis=0) + rng.uniform(-0.45, 0.45, size=(2 * N ** 3, 3)) / N]
